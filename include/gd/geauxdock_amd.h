@@ -1,0 +1,192 @@
+/*
+ * geauxdock_amd.h -- C ABI of the MI355X replica Monte Carlo docking library
+ * (libgeauxdock_amd.so).
+ *
+ * Every entry point returns 0 on success or a negative GD_ERR_* code; the
+ * message of the last failure on the calling thread is gd_last_error().
+ * Nothing here exits the process (the reference prints and calls
+ * exit(EXIT_FAILURE), e.g. src/frontend/frontend.cpp:381-402).
+ *
+ * Boundary replaced.  The reference's backend is the class
+ *   class Docker { Complex *complex; Record *record; Docker(); ~Docker(); void run(); }
+ * (src/backend_gpu/backend_gpu.h:10-23, src/backend_cpu_mic/backend_cpu_mic.h:8-16),
+ * whose run() loops the chunk kernel MonteCarlo_d(complex, record, s1, s2max)
+ * (src/backend_gpu/kernel_cuda_l1_montecarlo.cu:29-33,
+ *  src/backend_cpu_mic/kernel_cpp_montecarlo_soa.cpp:32-33) and reaches the
+ * empty replica-exchange hook between chunks (src/backend_gpu/backend_gpu.cu:106).
+ * Those calls map to gd_upload_target + gd_upload_job, gd_montecarlo,
+ * gd_exchange and gd_read_records below; include/gd/docker.hpp wraps them
+ * back into a Docker class.
+ */
+#ifndef GEAUXDOCK_AMD_H
+#define GEAUXDOCK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gd_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GD_OK = 0,
+    GD_ERR_ARG = -1,     /* bad argument or size */
+    GD_ERR_IO = -2,      /* input file missing or malformed */
+    GD_ERR_DEVICE = -3,  /* HIP runtime error */
+    GD_ERR_NOMEM = -4,   /* device memory budget exceeded */
+    GD_ERR_STATE = -5,   /* call out of order / unknown handle */
+    GD_ERR_BUSY = -6     /* serving: request not finished yet */
+};
+
+const char *gd_last_error(void);
+const char *gd_version(void);
+
+/* ------------------------------------------------------------------ */
+/* random streams                                                      */
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 block (Salmon et al., SC'11): out = philox(ctr, key). */
+void gd_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* The injected stream: 31-bit integer k in [0, 2^31-1] for
+ *   Monte Carlo draws (kind 0): replica idx_rep r, step s >= 1, slot j
+ *     slots 0..5 = move components, slot 6 = Metropolis accept;
+ *   exchange draws (kind 1): exchange round s, ladder r, pair slot j.
+ * The uniform used is (float)k / (float)RAND_MAX, exactly as the reference's
+ * MyRand_d (src/backend_cpu_mic/kernel_cpp_montecarlo_soa.cpp:8-15)
+ * converts rand_r(); on glibc (float)RAND_MAX == 2^31, so u = (float)k * 2^-31. */
+uint32_t gd_stream_k(uint64_t seed, uint32_t kind, uint32_t r, uint32_t s, uint32_t j);
+
+/* ------------------------------------------------------------------ */
+/* host loader and staging (no GPU needed)                             */
+/* ------------------------------------------------------------------ */
+typedef struct gd_raw_handle gd_raw_handle;
+typedef struct gd_complex_handle gd_complex_handle;
+
+/* Loader::load for one CSV job line (src/frontend/frontend.cpp:415-457):
+ * reads protein, parameters, ligand and .ff files named in the line. */
+int gd_load_job_line(const char *csv_line, gd_raw_handle **out);
+/* the same with explicit file paths and run parameters */
+int gd_load_job(const char *prt_path, const char *lig_path, const char *ff_path, const char *lig_id,
+                const char *weight_path, const char *enepara_path, const char *nora_path,
+                const char *norb_path, int n_tmp, float temp_high, float temp_low,
+                float tras_scale, float rot_scale, int n_dump, gd_raw_handle **out);
+const gd_Raw *gd_raw_view(const gd_raw_handle *h);
+void gd_raw_free(gd_raw_handle *h);
+
+/* Optimize* + set_temperature + set_replica (src/frontend/util_optimize_soa.cpp,
+ * src/frontend/frontend.cpp:222-299): raw -> staged tables. */
+int gd_stage(const gd_Raw *raw, gd_complex_handle **out);
+const gd_Complex *gd_complex_view(const gd_complex_handle *h);
+void gd_complex_free(gd_complex_handle *h);
+
+/* structure sizes and offsets as compiled (0: sizeof gd_Complex,
+ * 1: offsetof(gd_Complex, enepara), 2: sizeof gd_Raw, 3: offsetof(gd_Raw,
+ * enepara0), 4: sizeof gd_EnePara, 5: sizeof gd_EnePara0, 6: sizeof
+ * gd_ReplicaMC, 7: sizeof gd_Temp, 8: offsetof(gd_Complex, mcpara),
+ * 9: offsetof(gd_Raw, n_tmp)); lets bindings check their mirrors */
+size_t gd_abi_layout(int which);
+
+/* loader pieces for tests */
+double gd_superposed_rmsd(const float *ax, const float *ay, const float *az, const float *bx,
+                          const float *by, const float *bz, int n);
+
+/* ------------------------------------------------------------------ */
+/* device context                                                      */
+/* ------------------------------------------------------------------ */
+typedef struct gd_ctx gd_ctx;
+
+int gd_create(int device, gd_ctx **out);
+void gd_destroy(gd_ctx *ctx);
+/* HIP device properties of the context's GPU */
+int gd_device_info(gd_ctx *ctx, char *name, int name_len, int *cu_count, size_t *hbm_bytes);
+
+/* Protein conformations, EnePara and Psp of a complex, kept resident on the
+ * device for any number of jobs (Docker H2D of Complex, backend_gpu.cu:83,
+ * split so the target is uploaded once). */
+int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id);
+int gd_free_target(gd_ctx *ctx, int target_id);
+
+/* Ligand conformations, KDE, MCS, temperatures, replicas and move scales of
+ * a complex, docked against a resident target.  `seed` keys the job's
+ * injected random stream. */
+int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t seed, int *job_id);
+int gd_free_job(gd_ctx *ctx, int job_id);
+/* device bytes a job of these sizes takes (admission by HBM footprint) */
+size_t gd_job_footprint(int lig_natom, int n_lig, int kde_npoint, int mcs_entries, int mcs_nrow,
+                        int n_tmp, int n_rep, int record_cap);
+
+/* MonteCarlo_d chunk, plain form: steps s1 .. s1+s2max-1 of every replica
+ * of the job, one workgroup per replica, then a synchronous record read-back
+ * window (kernel_cuda_l1_montecarlo.cu:29-551, backend_gpu.cu:91-107).
+ * Records of the chunk: up to s2max accepted states per replica. */
+int gd_montecarlo(gd_ctx *ctx, int job_id, int s1, int s2max);
+
+/* Temperature replica exchange between neighbouring temperatures of every
+ * (ligand conformation, protein conformation) ladder; round `round` pairs
+ * (t, t+1) with t = round % 2, 2 + round % 2, ...; acceptance
+ * min(1, exp((beta_t - beta_t+1)(E_t - E_t+1))) in FP64 against the kind-1
+ * stream.  Accepted pairs swap temperature labels (ReplicaMC.idx_tmp).
+ * n_accept (optional) receives the number of accepted swaps. */
+int gd_exchange(gd_ctx *ctx, int job_id, int round, int *n_accept);
+
+/* Pose energies for parity: for pose i, replica rep_idx[i] with movematrix
+ * move[6i..6i+5]; writes the 10 normalised terms (vdw ele pmf psp hdb hpc
+ * kde lhm dst total) to e_out[10i..]. */
+int gd_energy(gd_ctx *ctx, int job_id, int n, const int *rep_idx, const float *move, float *e_out);
+
+int gd_read_replicas(gd_ctx *ctx, int job_id, gd_ReplicaMC *out);
+int gd_write_replicas(gd_ctx *ctx, int job_id, const gd_ReplicaMC *in);
+/* records of the last chunk: out[r * cap + k] for k < next_entry[r] */
+int gd_read_records(gd_ctx *ctx, int job_id, gd_ReplicaMC *out, int *next_entry, int cap);
+
+/* ------------------------------------------------------------------ */
+/* batched form: several jobs in one persistent fused launch per chunk */
+/* ------------------------------------------------------------------ */
+typedef struct gd_batch_opts {
+    int steps;             /* MC steps per job */
+    int exchange_interval; /* steps between exchange rounds, 0 = none */
+    int use_graph;         /* capture the chunk sequence in a hipGraph */
+    int slots_per_wave;    /* autotuned lane packing: 0 = auto */
+    int waves_per_group;   /* 0 = auto */
+    int point_slices;      /* 0 = auto */
+    int compensated;       /* Kahan-compensated per-replica accumulation */
+    int keep_records;      /* keep the accept records of the last chunk */
+} gd_batch_opts;
+
+int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
+                 double *elapsed_ms);
+
+/* ------------------------------------------------------------------ */
+/* serving: continuous batching of docking requests on one GPU          */
+/* ------------------------------------------------------------------ */
+typedef struct gd_server gd_server;
+typedef struct gd_server_opts {
+    size_t hbm_budget;     /* bytes of device memory jobs may take */
+    int max_active_jobs;   /* slots */
+    int chunk_steps;       /* MC steps per batched launch */
+    int exchange_interval; /* 0 = none */
+} gd_server_opts;
+
+typedef struct gd_request_result {
+    int status;            /* 0 done, GD_ERR_BUSY pending, other <0 failed/cancelled */
+    int n_rep;
+    double t_submit_ms, t_admit_ms, t_done_ms; /* server clock */
+} gd_request_result;
+
+int gd_server_create(gd_ctx *ctx, int target_id, const gd_server_opts *opts, gd_server **out);
+void gd_server_destroy(gd_server *srv);
+/* queue a request: the complex's job tables, steps, seed; returns an id */
+int gd_server_submit(gd_server *srv, const gd_Complex *cx, int steps, uint64_t seed, int *req_id);
+int gd_server_poll(gd_server *srv, int req_id, gd_request_result *res, gd_ReplicaMC *final_replicas);
+int gd_server_cancel(gd_server *srv, int req_id);
+/* run the serving loop until every queued request is finished */
+int gd_server_drain(gd_server *srv);
+double gd_server_clock_ms(gd_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GEAUXDOCK_AMD_H */
