@@ -1,0 +1,548 @@
+// Device half of the C ABI: context, resident target and job tables, the
+// plain Monte Carlo chunk (one workgroup per replica), pose energies and the
+// temperature exchange.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+namespace gd {
+
+#define GD_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) return fail(GD_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+hipError_t DevBuf::alloc(size_t n)
+{
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    else p = nullptr;
+    return e;
+}
+
+hipError_t upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s)
+{
+    if (b.bytes < bytes || !b.p) {
+        hipError_t e = b.alloc(bytes);
+        if (e != hipSuccess) return e;
+    }
+    if (bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
+}
+
+// Plain form: one workgroup per replica with the reference's 256-thread
+// block (TperB, src/GNUmakefile.common.mk:62-114): the ligand atoms fill
+// ceil(N_L / 64) waves and the remaining waves split the protein points.
+Shape plain_shape(int natom)
+{
+    Shape s;
+    s.nrep = 1;
+    s.slot_waves = (natom + 63) / 64;
+    s.slices = std::max(1, 4 / s.slot_waves);
+    s.waves = s.slot_waves * s.slices;
+    return s;
+}
+
+int ensure_records(gd_ctx *ctx, JobRes &j, int cap)
+{
+    if (j.rec_cap >= cap && j.rec.p) return GD_OK;
+    const size_t bytes = sizeof(gd_ReplicaMC) * (size_t)j.n_rep * cap;
+    GD_HIP(j.rec.alloc(bytes));
+    GD_HIP(hipMemsetAsync(j.rec.p, 0, bytes, ctx->stream));
+    j.rec_cap = cap;
+    j.hj.rec = j.rec.as<gd_ReplicaMC>();
+    j.hj.rec_cap = cap;
+    GD_HIP(hipMemcpyAsync(j.self.p, &j.hj, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
+    return GD_OK;
+}
+
+int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
+              int waves, int max_rep, bool compensated, bool records)
+{
+    GD_HIP(upload(ctx->task_buf, tasks.data(), sizeof(DevTask) * tasks.size(), ctx->stream));
+    GD_HIP(launch_mc(t.dt, t.sc, jobs_dev, ctx->task_buf.as<DevTask>(), (int)tasks.size(), waves, max_rep,
+                     compensated, records, ctx->stream, nullptr));
+    return GD_OK;
+}
+
+} // namespace gd
+
+using namespace gd;
+
+namespace {
+
+int check_ctx(gd_ctx *ctx)
+{
+    if (!ctx) return fail(GD_ERR_ARG, "null context");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(GD_ERR_DEVICE, "hipSetDevice failed");
+    return GD_OK;
+}
+
+JobRes *find_job(gd_ctx *ctx, int id)
+{
+    auto it = ctx->jobs.find(id);
+    return it == ctx->jobs.end() ? nullptr : &it->second;
+}
+
+} // namespace
+
+extern "C" {
+
+int gd_create(int device, gd_ctx **out)
+{
+    if (!out) return fail(GD_ERR_ARG, "null argument");
+    int n = 0;
+    GD_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(GD_ERR_ARG, "no such device");
+    GD_HIP(hipSetDevice(device));
+    gd_ctx *c = new gd_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return fail(GD_ERR_DEVICE, "hipGetDeviceProperties failed");
+    }
+    c->cu_count = prop.multiProcessorCount;
+    c->hbm_bytes = prop.totalGlobalMem;
+    c->name = prop.gcnArchName;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(GD_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    *out = c;
+    return GD_OK;
+}
+
+void gd_destroy(gd_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->jobs.clear();
+    ctx->targets.clear();
+    ctx->task_buf.release();
+    ctx->job_buf.release();
+    ctx->pose_rep.release();
+    ctx->pose_move.release();
+    ctx->pose_out.release();
+    ctx->counter.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int gd_device_info(gd_ctx *ctx, char *name, int name_len, int *cu_count, size_t *hbm_bytes)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (name && name_len > 0) {
+        std::strncpy(name, ctx->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (cu_count) *cu_count = ctx->cu_count;
+    if (hbm_bytes) *hbm_bytes = ctx->hbm_bytes;
+    return GD_OK;
+}
+
+int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!cx || !target_id) return fail(GD_ERR_ARG, "null argument");
+    const int np = cx->size.prt_npoint, nconf = cx->size.n_prt;
+    if (np <= 0 || nconf <= 0) return fail(GD_ERR_ARG, "empty protein");
+    const gd_EnePara &ep = cx->enepara;
+
+    // points re-sorted by type, keeping the class order within a type
+    std::vector<int> ord(np);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cx->prt_t[x] < cx->prt_t[y]; });
+    for (int i = 0; i < np; ++i)
+        if (cx->prt_t[i] < 0 || cx->prt_t[i] >= GD_MAXTP1) return fail(GD_ERR_ARG, "protein point type out of range");
+
+    // PSP rows of residues that carry at least one entry, for class-2 points
+    std::map<int, int> slot_of;
+    std::vector<float> pspc;
+    std::vector<float2> pa(np);
+    for (int i = 0; i < np; ++i) {
+        const int s = ord[i];
+        float2 v;
+        v.x = cx->prt_cdc[s] == 1 ? cx->prt_hpp[s] : 0.0f;
+        int slot = -1;
+        if (cx->prt_c[s] == 2) {
+            const int res = cx->prt_seq3r[s];
+            if (res < 0 || res >= cx->psp_width) return fail(GD_ERR_ARG, "residue serial outside the PSP table");
+            auto it = slot_of.find(res);
+            if (it != slot_of.end()) {
+                slot = it->second;
+            } else {
+                bool any = false;
+                for (int t = 0; t < GD_MAXTP2; ++t) any |= cx->psp[(size_t)t * cx->psp_width + res] != 0.0f;
+                if (any) {
+                    slot = (int)slot_of.size();
+                    slot_of[res] = slot;
+                    for (int t = 0; t < GD_MAXTP2; ++t) pspc.push_back(cx->psp[(size_t)t * cx->psp_width + res]);
+                }
+            }
+        }
+        int bits;
+        std::memcpy(&bits, &slot, 4);
+        std::memcpy(&v.y, &bits, 4);
+        pa[i] = v;
+    }
+    std::vector<float4> pt((size_t)nconf * np);
+    for (int c = 0; c < nconf; ++c)
+        for (int i = 0; i < np; ++i) {
+            const size_t s = (size_t)c * np + ord[i];
+            pt[(size_t)c * np + i] = make_float4(cx->prt_x[s], cx->prt_y[s], cx->prt_z[s], cx->prt_ele[ord[i]]);
+        }
+    std::vector<int4> seg;
+    for (int i = 0; i < np;) {
+        const int t = cx->prt_t[ord[i]];
+        int j = i;
+        while (j < np && cx->prt_t[ord[j]] == t) ++j;
+        seg.push_back(make_int4(t, i, j, 0));
+        i = j;
+    }
+    std::vector<float> pp((size_t)GD_MAXTP1 * GD_MAXTP2 * PP_N, 0.0f);
+    for (int tp = 0; tp < GD_MAXTP1; ++tp)
+        for (int tl = 0; tl < GD_MAXTP2; ++tl) {
+            float *q = &pp[((size_t)tp * GD_MAXTP2 + tl) * PP_N];
+            q[PP_P1A] = ep.p1a[tl][tp];
+            q[PP_P2A] = ep.p2a[tl][tp];
+            q[PP_PMF0] = ep.pmf0[tl][tp];
+            q[PP_PMF1] = ep.pmf1[tl][tp];
+            q[PP_HDB0] = ep.hdb0[tl][tp];
+            q[PP_HDB1] = ep.hdb0[tl][tp] > 0.1f ? ep.hdb1[tl][tp] : 0.0f;
+        }
+    if (pspc.empty()) pspc.assign(GD_MAXTP2, 0.0f);
+
+    const int id = ctx->next_id++;
+    TargetRes &t = ctx->targets[id];
+    GD_HIP(upload(t.pt, pt.data(), sizeof(float4) * pt.size(), ctx->stream));
+    GD_HIP(upload(t.pa, pa.data(), sizeof(float2) * pa.size(), ctx->stream));
+    GD_HIP(upload(t.pp, pp.data(), sizeof(float) * pp.size(), ctx->stream));
+    GD_HIP(upload(t.pspc, pspc.data(), sizeof(float) * pspc.size(), ctx->stream));
+    GD_HIP(upload(t.seg, seg.data(), sizeof(int4) * seg.size(), ctx->stream));
+    t.dt.pt = t.pt.as<float4>();
+    t.dt.pa = t.pa.as<float2>();
+    t.dt.pp = t.pp.as<float>();
+    t.dt.pspc = t.pspc.as<float>();
+    t.dt.seg = t.seg.as<int4>();
+    t.dt.npoint = np;
+    t.dt.npad = np;
+    t.dt.nseg = (int)seg.size();
+    t.dt.nconf = nconf;
+    for (int d = 0; d < 3; ++d) t.dt.pocket[d] = cx->prt_pocket_center[d];
+    DevScalars &s = t.sc;
+    s.lj0 = ep.lj0;
+    s.lj1 = ep.lj1;
+    s.el0 = ep.el0;
+    s.el1 = ep.el1;
+    s.a1 = ep.a1;
+    s.b1 = ep.b1;
+    s.inv_el1 = 1.0f / ep.el1;
+    s.kde2 = ep.kde2;
+    s.kde3inv = 1.0f / ep.kde3;
+    for (int i = 0; i < GD_MAXTP2; ++i) {
+        s.hpl0[i] = ep.hpl0[i];
+        s.hpl1[i] = ep.hpl1[i];
+        s.hpl2[i] = ep.hpl2[i];
+    }
+    for (int i = 0; i < GD_MAXWEI; ++i) {
+        s.a_para[i] = ep.a_para[i];
+        s.b_para[i] = ep.b_para[i];
+        s.w[i] = ep.w[i];
+    }
+    t.n_prt = nconf;
+    t.npoint = np;
+    for (const int4 &g : seg) {
+        t.seg_host.push_back(g.x);
+        t.seg_host.push_back(g.y);
+        t.seg_host.push_back(g.z);
+    }
+    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.pspc.bytes + t.seg.bytes;
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    *target_id = id;
+    return GD_OK;
+}
+
+int gd_free_target(gd_ctx *ctx, int target_id)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    for (auto &kv : ctx->jobs)
+        if (kv.second.target == target_id) return fail(GD_ERR_STATE, "target still has jobs");
+    if (!ctx->targets.erase(target_id)) return fail(GD_ERR_STATE, "unknown target");
+    return GD_OK;
+}
+
+size_t gd_job_footprint(int lig_natom, int n_lig, int kde_npoint, int mcs_entries, int mcs_nrow, int n_tmp,
+                        int n_rep, int record_cap)
+{
+    const size_t a = (size_t)lig_natom;
+    return 2 * sizeof(float4) * a * n_lig + (sizeof(int) + sizeof(int2)) * a + sizeof(float4) * kde_npoint +
+           sizeof(int) * (mcs_nrow + 1) + sizeof(float4) * mcs_entries + sizeof(float) * mcs_nrow +
+           sizeof(float) * n_tmp + sizeof(gd_ReplicaMC) * (size_t)n_rep * (1 + record_cap) +
+           sizeof(int) * (size_t)n_rep + sizeof(DevJob) + 9 * 256; // allocation granularity slack
+}
+
+int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t seed, int *job_id)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!cx || !job_id) return fail(GD_ERR_ARG, "null argument");
+    auto tit = ctx->targets.find(target_id);
+    if (tit == ctx->targets.end()) return fail(GD_ERR_STATE, "unknown target");
+    const gd_ComplexSize &z = cx->size;
+    if (z.lig_natom <= 0 || z.lig_natom > 1024) return fail(GD_ERR_ARG, "ligand atom count must be 1..1024");
+    if (z.n_lig <= 0 || z.n_tmp <= 0 || z.n_tmp > 64 || z.kde_npoint <= 0 || z.mcs_nrow <= 0)
+        return fail(GD_ERR_ARG, "bad job sizes (n_tmp must be 1..64)");
+    if (z.n_prt != tit->second.n_prt) return fail(GD_ERR_ARG, "job and target protein conformation counts differ");
+    if (z.prt_npoint != tit->second.npoint) return fail(GD_ERR_ARG, "job and target point counts differ");
+    if (z.n_rep != z.n_lig * z.n_prt * z.n_tmp) return fail(GD_ERR_ARG, "n_rep != n_lig * n_prt * n_tmp");
+    const int na = z.lig_natom;
+
+    std::vector<float4> lig3((size_t)z.n_lig * na), lig2((size_t)z.n_lig * na);
+    for (size_t i = 0; i < lig3.size(); ++i) {
+        lig3[i] = make_float4(cx->lig_x3[i], cx->lig_y3[i], cx->lig_z3[i], cx->lig_c3[i]);
+        lig2[i] = make_float4(cx->lig_x2[i], cx->lig_y2[i], cx->lig_z2[i], 0.0f);
+    }
+    std::vector<int> lt(cx->lig_t3, cx->lig_t3 + na);
+    for (int v : lt)
+        if (v < 0 || v >= GD_MAXTP2) return fail(GD_ERR_ARG, "ligand atom type out of range");
+    std::vector<int2> krange(na);
+    for (int i = 0; i < na; ++i) {
+        int b = -1, e = -1;
+        for (int k = 0; k < z.kde_npoint; ++k)
+            if (cx->kde_t[k] == lt[i]) {
+                if (b < 0) b = k;
+                e = k + 1;
+            }
+        if (b < 0) b = e = 0;
+        for (int k = b; k < e; ++k)
+            if (cx->kde_t[k] != lt[i]) return fail(GD_ERR_ARG, "KDE table is not sorted by type");
+        krange[i] = make_int2(b, e);
+    }
+    std::vector<float4> kde(z.kde_npoint);
+    for (int k = 0; k < z.kde_npoint; ++k) kde[k] = make_float4(cx->kde_x[k], cx->kde_y[k], cx->kde_z[k], 0.0f);
+    const int nent = cx->mcs_rowptr[z.mcs_nrow];
+    std::vector<int> mrow(cx->mcs_rowptr, cx->mcs_rowptr + z.mcs_nrow + 1);
+    std::vector<float4> ment(std::max(nent, 1));
+    for (int k = 0; k < nent; ++k) {
+        const int ai = cx->mcs_i[k];
+        if (ai < 0 || ai >= na) return fail(GD_ERR_ARG, "MCS atom index out of range");
+        float w;
+        std::memcpy(&w, &ai, 4);
+        ment[k] = make_float4(cx->mcs_x[k], cx->mcs_y[k], cx->mcs_z[k], w);
+    }
+    for (int m = 0; m < z.mcs_nrow; ++m)
+        if (mrow[m + 1] <= mrow[m]) return fail(GD_ERR_ARG, "empty MCS row");
+    std::vector<float> mtcc(cx->mcs_tcc, cx->mcs_tcc + z.mcs_nrow);
+    std::vector<float> mbeta(z.n_tmp);
+    for (int t = 0; t < z.n_tmp; ++t) mbeta[t] = cx->temp[t].minus_beta;
+
+    const int id = ctx->next_id++;
+    JobRes &j = ctx->jobs[id];
+    j.target = target_id;
+    hipStream_t s = ctx->stream;
+    GD_HIP(upload(j.lig3, lig3.data(), sizeof(float4) * lig3.size(), s));
+    GD_HIP(upload(j.lig2, lig2.data(), sizeof(float4) * lig2.size(), s));
+    GD_HIP(upload(j.lt, lt.data(), sizeof(int) * lt.size(), s));
+    GD_HIP(upload(j.krange, krange.data(), sizeof(int2) * krange.size(), s));
+    GD_HIP(upload(j.kde, kde.data(), sizeof(float4) * kde.size(), s));
+    GD_HIP(upload(j.mrow, mrow.data(), sizeof(int) * mrow.size(), s));
+    GD_HIP(upload(j.ment, ment.data(), sizeof(float4) * ment.size(), s));
+    GD_HIP(upload(j.mtcc, mtcc.data(), sizeof(float) * mtcc.size(), s));
+    GD_HIP(upload(j.mbeta, mbeta.data(), sizeof(float) * mbeta.size(), s));
+    GD_HIP(upload(j.rep, cx->replica, sizeof(gd_ReplicaMC) * z.n_rep, s));
+    GD_HIP(j.next_entry.alloc(sizeof(int) * z.n_rep));
+    GD_HIP(hipMemsetAsync(j.next_entry.p, 0, sizeof(int) * z.n_rep, s));
+    GD_HIP(j.self.alloc(sizeof(DevJob)));
+    DevJob &h = j.hj;
+    h.lig3 = j.lig3.as<float4>();
+    h.lig2 = j.lig2.as<float4>();
+    h.lt = j.lt.as<int>();
+    h.krange = j.krange.as<int2>();
+    h.kde = j.kde.as<float4>();
+    h.mrow = j.mrow.as<int>();
+    h.ment = j.ment.as<float4>();
+    h.mtcc = j.mtcc.as<float>();
+    h.mbeta = j.mbeta.as<float>();
+    h.rep = j.rep.as<gd_ReplicaMC>();
+    h.rec = nullptr;
+    h.next_entry = j.next_entry.as<int>();
+    h.natom = na;
+    h.nrow = z.mcs_nrow;
+    h.rec_cap = 0;
+    h.n_rep = z.n_rep;
+    h.n_lig = z.n_lig;
+    h.n_prt = z.n_prt;
+    h.n_tmp = z.n_tmp;
+    for (int d = 0; d < 3; ++d) h.center[d] = cx->lig_center[d];
+    for (int k = 0; k < 6; ++k) h.scale[k] = cx->mcpara.move_scale[k];
+    h.key0 = (uint32_t)seed;
+    h.key1 = (uint32_t)(seed >> 32);
+    GD_HIP(hipMemcpyAsync(j.self.p, &h, sizeof(DevJob), hipMemcpyHostToDevice, s));
+    j.n_rep = z.n_rep;
+    j.n_lig = z.n_lig;
+    j.n_prt = z.n_prt;
+    j.n_tmp = z.n_tmp;
+    j.natom = na;
+    j.nrow = z.mcs_nrow;
+    j.seed = seed;
+    j.rep_prt.resize(z.n_rep);
+    for (int r = 0; r < z.n_rep; ++r) {
+        j.rep_prt[r] = cx->replica[r].idx_prt;
+        if (j.rep_prt[r] < 0 || j.rep_prt[r] >= z.n_prt || cx->replica[r].idx_lig < 0 ||
+            cx->replica[r].idx_lig >= z.n_lig || cx->replica[r].idx_tmp < 0 || cx->replica[r].idx_tmp >= z.n_tmp) {
+            ctx->jobs.erase(id);
+            return fail(GD_ERR_ARG, "replica index fields out of range");
+        }
+    }
+    j.bytes = j.lig3.bytes + j.lig2.bytes + j.lt.bytes + j.krange.bytes + j.kde.bytes + j.mrow.bytes + j.ment.bytes +
+              j.mtcc.bytes + j.mbeta.bytes + j.rep.bytes + j.next_entry.bytes + j.self.bytes;
+    GD_HIP(hipStreamSynchronize(s));
+    *job_id = id;
+    return GD_OK;
+}
+
+int gd_free_job(gd_ctx *ctx, int job_id)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (!ctx->jobs.erase(job_id)) return fail(GD_ERR_STATE, "unknown job");
+    return GD_OK;
+}
+
+int gd_montecarlo(gd_ctx *ctx, int job_id, int s1, int s2max)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j) return fail(GD_ERR_STATE, "unknown job");
+    if (s1 < 0 || s2max <= 0) return fail(GD_ERR_ARG, "bad step range");
+    if (int rc = ensure_records(ctx, *j, std::max(s2max, GD_STEPS_PER_DUMP))) return rc;
+    const TargetRes &t = ctx->targets.at(j->target);
+    const Shape sh = plain_shape(j->natom);
+    std::vector<DevTask> tasks(j->n_rep);
+    for (int r = 0; r < j->n_rep; ++r)
+        tasks[r] = DevTask{0, r, 1, j->rep_prt[r], s1, s2max, sh.slices, sh.slot_waves};
+    if (int rc = run_tasks(ctx, t, j->self.as<DevJob>(), tasks, sh.waves, 1, false, true)) return rc;
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    return GD_OK;
+}
+
+int gd_exchange(gd_ctx *ctx, int job_id, int round, int *n_accept)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j) return fail(GD_ERR_STATE, "unknown job");
+    if (round < 0) return fail(GD_ERR_ARG, "negative exchange round");
+    if (!ctx->counter.p) GD_HIP(ctx->counter.alloc(sizeof(int)));
+    GD_HIP(hipMemsetAsync(ctx->counter.p, 0, sizeof(int), ctx->stream));
+    GD_HIP(launch_exchange(j->self.as<DevJob>(), j->n_lig * j->n_prt, j->n_lig, j->n_tmp, round,
+                           ctx->counter.as<int>(), ctx->stream, nullptr));
+    int cnt = 0;
+    GD_HIP(hipMemcpyAsync(&cnt, ctx->counter.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    if (n_accept) *n_accept = cnt;
+    return GD_OK;
+}
+
+int gd_energy(gd_ctx *ctx, int job_id, int n, const int *rep_idx, const float *move, float *e_out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j) return fail(GD_ERR_STATE, "unknown job");
+    if (n < 0 || (n > 0 && (!rep_idx || !move || !e_out))) return fail(GD_ERR_ARG, "bad pose arguments");
+    if (n == 0) return GD_OK;
+    for (int i = 0; i < n; ++i)
+        if (rep_idx[i] < 0 || rep_idx[i] >= j->n_rep) return fail(GD_ERR_ARG, "replica index out of range");
+    const TargetRes &t = ctx->targets.at(j->target);
+    const Shape sh = plain_shape(j->natom);
+    std::vector<DevTask> tasks(n);
+    for (int i = 0; i < n; ++i) tasks[i] = DevTask{0, i, 1, j->rep_prt[rep_idx[i]], 0, 1, sh.slices, sh.slot_waves};
+    hipStream_t s = ctx->stream;
+    GD_HIP(upload(ctx->task_buf, tasks.data(), sizeof(DevTask) * n, s));
+    GD_HIP(upload(ctx->pose_rep, rep_idx, sizeof(int) * n, s));
+    GD_HIP(upload(ctx->pose_move, move, sizeof(float) * 6 * n, s));
+    if (ctx->pose_out.bytes < sizeof(float) * GD_MAXWEI * n) GD_HIP(ctx->pose_out.alloc(sizeof(float) * GD_MAXWEI * n));
+    GD_HIP(launch_energy(t.dt, t.sc, j->self.as<DevJob>(), ctx->task_buf.as<DevTask>(), n, sh.waves, false,
+                         ctx->pose_rep.as<int>(), ctx->pose_move.as<float>(), ctx->pose_out.as<float>(), s));
+    GD_HIP(hipMemcpyAsync(e_out, ctx->pose_out.p, sizeof(float) * GD_MAXWEI * n, hipMemcpyDeviceToHost, s));
+    GD_HIP(hipStreamSynchronize(s));
+    return GD_OK;
+}
+
+int gd_read_replicas(gd_ctx *ctx, int job_id, gd_ReplicaMC *out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j || !out) return fail(GD_ERR_ARG, "unknown job or null output");
+    GD_HIP(hipMemcpyAsync(out, j->rep.p, sizeof(gd_ReplicaMC) * j->n_rep, hipMemcpyDeviceToHost, ctx->stream));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    return GD_OK;
+}
+
+int gd_write_replicas(gd_ctx *ctx, int job_id, const gd_ReplicaMC *in)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j || !in) return fail(GD_ERR_ARG, "unknown job or null input");
+    for (int r = 0; r < j->n_rep; ++r)
+        if (in[r].idx_prt != j->rep_prt[r] || in[r].idx_tmp < 0 || in[r].idx_tmp >= j->n_tmp || in[r].idx_lig < 0 ||
+            in[r].idx_lig >= j->n_lig)
+            return fail(GD_ERR_ARG, "replica index fields out of range or protein conformation changed");
+    // every (ligand conformation, protein conformation) ladder must hold each
+    // temperature exactly once (the exchange kernel indexes by it)
+    for (int p = 0; p < j->n_prt; ++p)
+        for (int l = 0; l < j->n_lig; ++l) {
+            std::vector<int> seen(j->n_tmp, 0);
+            for (int t = 0; t < j->n_tmp; ++t) {
+                const gd_ReplicaMC &r = in[j->n_tmp * j->n_lig * p + j->n_lig * t + l];
+                if (r.idx_lig != l || seen[r.idx_tmp]++)
+                    return fail(GD_ERR_ARG, "ladder temperatures are not a permutation");
+            }
+        }
+    GD_HIP(hipMemcpyAsync(j->rep.p, in, sizeof(gd_ReplicaMC) * j->n_rep, hipMemcpyHostToDevice, ctx->stream));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    return GD_OK;
+}
+
+int gd_read_records(gd_ctx *ctx, int job_id, gd_ReplicaMC *out, int *next_entry, int cap)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j) return fail(GD_ERR_STATE, "unknown job");
+    if (!j->rec.p) return fail(GD_ERR_STATE, "no records yet");
+    std::vector<int> ne(j->n_rep);
+    GD_HIP(hipMemcpyAsync(ne.data(), j->next_entry.p, sizeof(int) * j->n_rep, hipMemcpyDeviceToHost, ctx->stream));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    if (next_entry) std::copy(ne.begin(), ne.end(), next_entry);
+    if (out) {
+        if (cap < j->rec_cap) {
+            for (int r = 0; r < j->n_rep; ++r)
+                if (ne[r] > cap) return fail(GD_ERR_ARG, "record capacity too small");
+            for (int r = 0; r < j->n_rep; ++r)
+                GD_HIP(hipMemcpyAsync(out + (size_t)r * cap, j->rec.as<gd_ReplicaMC>() + (size_t)r * j->rec_cap,
+                                      sizeof(gd_ReplicaMC) * std::max(ne[r], 0), hipMemcpyDeviceToHost, ctx->stream));
+        } else {
+            GD_HIP(hipMemcpy2DAsync(out, sizeof(gd_ReplicaMC) * cap, j->rec.p, sizeof(gd_ReplicaMC) * j->rec_cap,
+                                    sizeof(gd_ReplicaMC) * j->rec_cap, j->n_rep, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        GD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return GD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,95 @@
+// Device context: resident targets and jobs, their device buffers and the
+// host mirrors the launchers need.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "gd/geauxdock_amd.h"
+
+namespace gd {
+
+int fail(int code, const std::string &msg);
+
+// owning device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release();
+    hipError_t alloc(size_t n);
+    template <class T> T *as() const { return (T *)p; }
+};
+
+struct TargetRes {
+    DevBuf pt, pa, pp, pspc, seg;
+    DevTarget dt{};
+    DevScalars sc{};
+    int n_prt = 0, npoint = 0;
+    std::vector<int> seg_host; // (type, begin, end) triples
+    size_t bytes = 0;
+};
+
+struct JobRes {
+    int target = -1;
+    DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, rep, rec, next_entry, self;
+    DevJob hj{};       // host copy of the descriptor
+    int n_rep = 0, n_lig = 0, n_prt = 0, n_tmp = 0, natom = 0, nrow = 0;
+    std::vector<int> rep_prt; // protein conformation of every replica (never changes)
+    int rec_cap = 0;
+    uint64_t seed = 0;
+    size_t bytes = 0;
+    int steps_done = 0;      // steps advanced by batched runs
+    int exchange_rounds = 0; // exchange rounds done by batched runs
+};
+
+// Workgroup shape of one task: R replicas x N_L atoms in slot_waves waves,
+// times `slices` protein-point slices.
+struct Shape {
+    int nrep, slot_waves, slices, waves;
+};
+
+Shape plain_shape(int natom);
+// replica packing for a W-wave workgroup: best lane use, then fewest slot waves
+Shape heuristic_shape(int natom, int W, int max_rep_block);
+// tasks of one job: consecutive replicas of each protein conformation, R per task
+void build_tasks(const JobRes &j, int job_slot, const Shape &sh, int s1, int s2max, std::vector<DevTask> &out);
+
+} // namespace gd
+
+struct gd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    size_t hbm_bytes = 0;
+    std::string name;
+    std::map<int, gd::TargetRes> targets;
+    std::map<int, gd::JobRes> jobs;
+    int next_id = 1;
+    // launch scratch reused across calls
+    gd::DevBuf task_buf, job_buf, pose_rep, pose_move, pose_out, counter;
+};
+
+namespace gd {
+hipError_t upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s);
+int ensure_records(gd_ctx *ctx, JobRes &j, int cap);
+int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
+              int waves, int max_rep, bool compensated, bool records);
+} // namespace gd

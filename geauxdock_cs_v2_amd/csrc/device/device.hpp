@@ -1,0 +1,108 @@
+// Device-side data layout shared by the kernels and the host code that
+// uploads and launches them.
+//
+// HBM layout (one resident target, any number of jobs):
+//   target  pt   float4[nconf][npad]  {x, y, z, ele} of every effective point,
+//                                     re-sorted by point type (stable within a
+//                                     type) so a run of points shares one set
+//                                     of type-pair parameters
+//           pa   float2[npad]         {hpc weight = cdc ? hpp : 0, PSP slot}
+//           seg  int4[nseg]           {point type, begin, end, 0}
+//           pp   float[31][24][8]     type-pair parameters, see PairParam
+//           pspc float[nslot][24]     PSP rows of residues that have entries
+//   job     lig3 float4[nlig][natom]  {x3, y3, z3, c3} type-sorted atoms
+//           lig2 float4[nlig][natom]  {x2, y2, z2, 0} file-order atoms (MCS)
+//           lt   int[natom]           t3
+//           krange int2[natom]        same-type range in the sorted KDE table
+//           kde  float4[nkde]         {x, y, z, 0}
+//           mrow int[nrow+1], ment float4[nent] {x, y, z, file atom index}, mtcc
+//           rep  gd_ReplicaMC[n_rep], rec gd_ReplicaMC[n_rep][rec_cap], next_entry
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "gd/gd_types.h"
+
+namespace gd {
+
+// per (point type, ligand type) parameters, 8 floats (32 bytes)
+enum PairParam {
+    PP_P1A = 0,  // LJ repulsive prefactor p1a
+    PP_P2A = 1,  // LJ attractive prefactor p2a
+    PP_PMF0 = 2, // contact distance
+    PP_PMF1 = 3, // contact depth
+    PP_HDB0 = 4, // H-bond distance
+    PP_HDB1 = 5, // H-bond inverse width, 0 where the pair has no H-bond term
+    PP_N = 8
+};
+
+struct DevScalars {
+    float lj0, lj1, el0, el1, a1, b1, inv_el1, kde2, kde3inv;
+    float hpl0[GD_MAXTP2], hpl1[GD_MAXTP2], hpl2[GD_MAXTP2];
+    float a_para[GD_MAXWEI], b_para[GD_MAXWEI], w[GD_MAXWEI];
+};
+
+struct DevTarget {
+    const float4 *pt;
+    const float2 *pa;
+    const float *pp;
+    const float *pspc;
+    const int4 *seg;
+    int npoint, npad, nseg, nconf;
+    float pocket[3];
+    int pad;
+};
+
+struct DevJob {
+    const float4 *lig3;
+    const float4 *lig2;
+    const int *lt;
+    const int2 *krange;
+    const float4 *kde;
+    const int *mrow;
+    const float4 *ment;
+    const float *mtcc;
+    const float *mbeta;
+    gd_ReplicaMC *rep;
+    gd_ReplicaMC *rec;
+    int *next_entry;
+    int natom, nrow, rec_cap, n_rep;
+    int n_lig, n_prt, n_tmp, pad;
+    float center[3];
+    float scale[6];
+    uint32_t key0, key1;
+};
+
+// A task is the set of replicas one workgroup advances: replicas
+// [rep0, rep0 + nrep) of job `job`, all in protein conformation `prt`,
+// stepping s1 .. s1 + s2max - 1.  In energy mode rep0 indexes the pose list.
+struct DevTask {
+    int job, rep0, nrep, prt;
+    int s1, s2max, slices, slot_waves;
+};
+
+enum KernelMode { MODE_MC = 0, MODE_ENERGY = 1 };
+
+// LDS bytes a workgroup of `waves` waves needs for `nrep` replicas
+size_t mc_lds_bytes(int waves, int nrep);
+
+// Launch the fused Monte Carlo kernel over `ntask` tasks, one workgroup of
+// `waves` waves each (every task's slot_waves * slices == waves).
+// step_base (device int, may be null) is added to every task's s1, so a
+// captured graph can advance the step counter without new kernel arguments.
+hipError_t launch_mc(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
+                     int ntask, int waves, int max_rep, bool compensated, bool keep_records,
+                     hipStream_t stream, const int *step_base);
+
+// Energy mode: poses i = task.rep0 (nrep == 1), replica pose_rep[i], move pose_move[6i..]
+hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
+                         int ntask, int waves, bool compensated, const int *pose_rep, const float *pose_move,
+                         float *pose_out, hipStream_t stream);
+
+// One exchange round over every ladder of a job.
+// round_base (device int, may be null) is added to `round`.
+hipError_t launch_exchange(const DevJob *job_dev, int n_ladder, int n_lig, int n_tmp, int round,
+                           int *accept_count, hipStream_t stream, const int *round_base);
+
+} // namespace gd

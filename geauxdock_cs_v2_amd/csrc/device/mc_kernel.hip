@@ -1,0 +1,476 @@
+// Fused replica Monte Carlo kernel for gfx950 (CDNA4, wave64).
+//
+// One workgroup advances one task: R replicas of one job that share a
+// protein conformation, for s2max steps (the chunk MonteCarlo_d runs,
+// src/backend_cpu_mic/kernel_cpp_montecarlo_soa.cpp:32-420).  Replica state
+// lives in LDS for the whole chunk.
+//
+// Lane mapping.  The R replicas' ligand atoms are laid out as R * N_L
+// "slots"; slot wave sw holds slots [64 sw, 64 sw + 64).  The workgroup has
+// slot_waves * slices waves: wave w serves slot wave w % slot_waves and
+// protein-point slice w / slot_waves.  Every lane of a wave walks the same
+// protein points, so point data is wave-uniform; each lane keeps its own
+// ligand atom.  Points are sorted by type, so the six type-pair parameters
+// of a lane are loaded once per type segment, not per pair.
+//
+// One step, per slot: proposal move from the injected stream, rotation and
+// translation of the atom, the six pair terms over the slice's points
+// (src/backend_cpu_mic/energy_prt_v1.cpp), cross-slice sum, per-atom
+// hydrophobic restraint, KDE over the atom type's KDE run
+// (energy_kde_v1.cpp) and the MCS rows assigned to the atom's lane
+// (energy_mcs_v3_ell.cpp); then a per-replica sum, normalisation, weighted
+// total, Metropolis accept and record append
+// (kernel_cpp_montecarlo_soa.cpp:333-390) by one thread per replica.
+#include "device.hpp"
+
+namespace gd {
+
+namespace {
+
+__device__ __forceinline__ void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                         uint32_t k1)
+{
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// (float)k / RAND_MAX with k = word >> 1; (float)RAND_MAX == 2^31
+__device__ __forceinline__ float uniform31(uint32_t w) { return (float)(w >> 1) * 0x1p-31f; }
+
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
+template <bool KAHAN>
+__device__ __forceinline__ void acc_add(float &s, float &c, float v)
+{
+    if (KAHAN) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    } else {
+        s += v;
+    }
+}
+
+struct Slot {
+    float x, y, z; // transformed atom
+    int tl;        // ligand type
+};
+
+} // namespace
+
+size_t mc_lds_bytes(int waves, int nrep)
+{
+    const int lanes = waves * 64;
+    // one region for the cross-slice partials and, after a barrier, the
+    // per-slot terms (lanes * 8 floats bounds both); then rsum, st, prop
+    // (16 + 16 + 8 floats per replica) and 4 ints per replica
+    return sizeof(float) * ((size_t)lanes * 8 + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * 4;
+}
+
+template <bool KAHAN, int MODE>
+__global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, const DevJob *__restrict__ jobs,
+                                                  const DevTask *__restrict__ tasks,
+                                                  const int *__restrict__ step_base, int keep_records,
+                                                  const int *__restrict__ pose_rep,
+                                                  const float *__restrict__ pose_move, float *__restrict__ pose_out)
+{
+    extern __shared__ float lds[];
+    const DevTask task = tasks[blockIdx.x];
+    const DevJob &J = jobs[task.job];
+    const int W = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int SW = task.slot_waves;
+    const int S = task.slices;
+    const int sw = wave % SW;
+    const int slice = wave / SW;
+    const int NL = J.natom;
+    const int R = task.nrep;
+    const int nslots = R * NL;
+    const int slot = sw * 64 + lane;
+    const bool active = slot < nslots;
+    const int g = active ? slot / NL : 0;
+    const int a = active ? slot - g * NL : 0;
+    const int lanes = W * 64;
+
+    float *red = lds;                      // [(S-1) * SW*64 * 6], then reused:
+    float *vals = lds;                     // [SW*64 * 8]
+    float *rsum = lds + (size_t)lanes * 8; // [R][16] sums + compensations
+    float *st = rsum + (size_t)R * 16;     // [R][16] movematrix[6], energy[10]
+    float *prop = st + (size_t)R * 16;     // [R][8] proposed move, accept uniform
+    int *ist = (int *)(prop + (size_t)R * 8); // [R][4] next_entry, is_accept, step, idx_tmp
+
+    const int s1 = task.s1 + (step_base ? *step_base : 0);
+    const int s2max = task.s2max;
+
+    if (MODE == MODE_MC) {
+        for (int i = threadIdx.x; i < R; i += blockDim.x) {
+            const gd_ReplicaMC &rp = J.rep[task.rep0 + i];
+            for (int k = 0; k < 6; ++k) st[i * 16 + k] = rp.movematrix[k];
+            for (int k = 0; k < GD_MAXWEI; ++k) st[i * 16 + 6 + k] = rp.energy[k];
+            ist[i * 4 + 0] = 0;
+            ist[i * 4 + 1] = rp.is_accept;
+            ist[i * 4 + 2] = rp.step;
+            ist[i * 4 + 3] = rp.idx_tmp;
+        }
+    }
+    __syncthreads();
+
+    // this lane's replica and atom
+    int rid, pose = 0;
+    if (MODE == MODE_MC) {
+        rid = task.rep0 + g;
+    } else {
+        pose = task.rep0;
+        rid = pose_rep[pose];
+    }
+    const int idx_lig = J.rep[rid].idx_lig;
+    const uint32_t idx_rep = (uint32_t)J.rep[rid].idx_rep;
+    const float4 L3 = J.lig3[(size_t)idx_lig * NL + a];
+    const int tl = J.lt[a];
+    const float lc = L3.w;
+    const int prt = task.prt;
+    const float4 *__restrict__ ptc = tg.pt + (size_t)prt * tg.npad;
+    const int np = tg.npoint;
+    const int lo = (int)(((long long)np * slice) / S);
+    const int hi = (int)(((long long)np * (slice + 1)) / S);
+
+    for (int s2 = 0; s2 < s2max; ++s2) {
+        const int step = s1 + s2;
+        float m[6];
+        float uacc = 0.0f;
+        if (MODE == MODE_MC) {
+            for (int k = 0; k < 6; ++k) m[k] = st[g * 16 + k];
+            if (step != 0) {
+                uint32_t b0 = (uint32_t)step, b1 = idx_rep, b2 = 0u, b3 = 0u;
+                uint32_t c0 = (uint32_t)step, c1 = idx_rep, c2 = 0u, c3 = 1u;
+                philox10(b0, b1, b2, b3, J.key0, J.key1);
+                philox10(c0, c1, c2, c3, J.key0, J.key1);
+                const float u[7] = {uniform31(b0), uniform31(b1), uniform31(b2), uniform31(b3),
+                                    uniform31(c0), uniform31(c1), uniform31(c2)};
+                // same rounding as the reference's scale * u + m (no contraction)
+                for (int k = 0; k < 6; ++k) m[k] = __fadd_rn(__fmul_rn(J.scale[k], u[k]), m[k]);
+                uacc = u[6];
+            }
+        } else {
+            for (int k = 0; k < 6; ++k) m[k] = pose_move[(size_t)pose * 6 + k];
+        }
+        if (a == 0 && slice == 0 && active) {
+            for (int k = 0; k < 6; ++k) prop[g * 8 + k] = m[k];
+            prop[g * 8 + 6] = uacc;
+        }
+
+        // Euler rotation of the cumulative angles
+        // (kernel_cpp_montecarlo_soa.cpp:241-255)
+        float s1a, c1a, s2a, c2a, s3a, c3a;
+        sincosf(m[3], &s1a, &c1a);
+        sincosf(m[4], &s2a, &c2a);
+        sincosf(m[5], &s3a, &c3a);
+        const float r00 = c1a * c2a, r01 = c1a * s2a * s3a - c3a * s1a, r02 = s1a * s3a + c1a * c3a * s2a;
+        const float r10 = c2a * s1a, r11 = c1a * c3a + s1a * s2a * s3a, r12 = c3a * s1a * s2a - c1a * s3a;
+        const float r20 = -s2a, r21 = c2a * s3a, r22 = c2a * c3a;
+        const float tx = m[0] + J.center[0], ty = m[1] + J.center[1], tz = m[2] + J.center[2];
+        Slot me;
+        me.x = r00 * L3.x + r01 * L3.y + r02 * L3.z + tx;
+        me.y = r10 * L3.x + r11 * L3.y + r12 * L3.z + ty;
+        me.z = r20 * L3.x + r21 * L3.y + r22 * L3.z + tz;
+        me.tl = tl;
+
+        // ---- pair terms over this slice's points ----
+        float evdw = 0.f, eele = 0.f, epmf = 0.f, epsp = 0.f, ehdb = 0.f, ehpc = 0.f;
+        float cvdw = 0.f, cele = 0.f, cpmf = 0.f, cpsp = 0.f, chdb = 0.f, chpc = 0.f;
+        for (int si = 0; si < tg.nseg; ++si) {
+            const int4 sg = tg.seg[si];
+            const int b = max(sg.y, lo), e = min(sg.z, hi);
+            if (b >= e) continue;
+            const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
+            const float4 pA = *(const float4 *)P;
+            const float4 pB = *(const float4 *)(P + 4);
+            const float p1a = pA.x, p2a = pA.y, pmf0 = pA.z, pmf1 = pA.w, hdb0 = pB.x, hdb1 = pB.y;
+            float sv = 0.f, se = 0.f, sp = 0.f, ss = 0.f, sh = 0.f, sq = 0.f;
+            for (int p = b; p < e; ++p) {
+                const float4 P4 = ptc[p];
+                const float2 A = tg.pa[p];
+                const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float rinv = __builtin_amdgcn_rsqf(d2);
+                const float d = d2 * rinv;
+
+                // hydrophobic contact (energy_prt_v1.cpp:44-50)
+                if (A.x != 0.0f) {
+                    const float u3 = fmaf(-d2, 0.5f / 81.0f / 81.0f / 81.0f / 81.0f, 2.5f / 81.0f / 81.0f / 81.0f);
+                    const float u2 = fmaf(-d2, u3, 4.5f / 81.0f / 81.0f);
+                    const float u1 = fmaf(-d2, u2, 3.5f / 81.0f);
+                    const float poly = fmaf(-d2, u1, 1.0f);
+                    sq = fmaf(A.x, d2 <= 81.0f ? poly : 0.0f, sq);
+                }
+                // soft-core Lennard-Jones (:62-71)
+                const float r2 = rinv * rinv;
+                const float r3 = r2 * rinv;
+                const float r6 = r3 * r3;
+                const float pp1 = p1a * (r6 * r3);
+                const float pp2 = p2a * r6;
+                const float pp4 = fmaf(pp1 * sc.lj0, fmaf(sc.lj1, d2, 1.0f), 1.0f);
+                sv = fmaf(pp1 - pp2, __builtin_amdgcn_rcpf(pp4), sv);
+                // electrostatics (:79-85); charge of the atom applied after the loop
+                const float s1e = sc.el1 * d;
+                const float gpoly = fmaf(s1e * s1e, fmaf(sc.b1, s1e, sc.a1), sc.el0);
+                const float g1 = s1e < 1.0f ? gpoly : rinv * sc.inv_el1;
+                se = fmaf(P4.w, g1, se);
+                // contact potential (:95-98)
+                const float dm = d - pmf0;
+                sp = fmaf(pmf1, __builtin_amdgcn_rcpf(1.0f + fast_exp(fmaf(-0.5f, d, 6.0f) * dm)), sp);
+                // pocket-specific term (:107-109)
+                const int pslot = __float_as_int(A.y);
+                if (pslot >= 0) {
+                    if (dm <= 0.0f) ss += tg.pspc[pslot * GD_MAXTP2 + me.tl];
+                }
+                // hydrogen bond (:138-143); hdb1 == 0 where the pair has none
+                const float h3 = (d - hdb0) * hdb1;
+                sh = fmaf(hdb1, fast_exp(-0.5f * h3 * h3), sh);
+            }
+            acc_add<KAHAN>(evdw, cvdw, sv);
+            acc_add<KAHAN>(eele, cele, se);
+            acc_add<KAHAN>(epmf, cpmf, sp);
+            acc_add<KAHAN>(epsp, cpsp, ss);
+            acc_add<KAHAN>(ehdb, chdb, sh);
+            acc_add<KAHAN>(ehpc, chpc, sq);
+        }
+
+        // ---- cross-slice sum ----
+        if (S > 1 && slice > 0) {
+            float *r = red + ((size_t)(slice - 1) * SW * 64 + slot) * 6;
+            r[0] = evdw;
+            r[1] = eele;
+            r[2] = epmf;
+            r[3] = epsp;
+            r[4] = ehdb;
+            r[5] = ehpc;
+        }
+        __syncthreads();
+        if (slice == 0 && active) {
+            for (int s = 1; s < S; ++s) {
+                const float *r = red + ((size_t)(s - 1) * SW * 64 + slot) * 6;
+                acc_add<KAHAN>(evdw, cvdw, r[0]);
+                acc_add<KAHAN>(eele, cele, r[1]);
+                acc_add<KAHAN>(epmf, cpmf, r[2]);
+                acc_add<KAHAN>(epsp, cpsp, r[3]);
+                acc_add<KAHAN>(ehdb, chdb, r[4]);
+                acc_add<KAHAN>(ehpc, chpc, r[5]);
+            }
+        }
+        __syncthreads(); // the partials region is reused for the per-slot terms
+        if (slice == 0 && active) {
+            // hydrophobic restraint of the atom (energy_prt_v1.cpp:154-155)
+            const float hpc2 = (ehpc - sc.hpl0[tl]) / sc.hpl1[tl];
+            const float hpc_atom = 0.5f * hpc2 * hpc2 - sc.hpl2[tl];
+
+            // KDE over the atom type's run of the type-sorted table
+            // (energy_kde_v1.cpp:1-77)
+            const int2 kr = J.krange[a];
+            float ks = 0.f;
+            for (int k = kr.x; k < kr.y; ++k) {
+                const float4 K4 = J.kde[k];
+                const float dx = me.x - K4.x, dy = me.y - K4.y, dz = me.z - K4.z;
+                ks += fast_exp(sc.kde2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+            }
+            const float kde_atom = kr.y > kr.x ? ks / (float)(kr.y - kr.x) : 0.0f;
+
+            // MCS rows a, a + N_L, ... of this replica over the file-order
+            // atom set (energy_mcs_v3_ell.cpp)
+            float mcs = 0.f;
+            for (int row = a; row < J.nrow; row += NL) {
+                const int rb = J.mrow[row], re = J.mrow[row + 1];
+                float s = 0.f;
+                for (int k = rb; k < re; ++k) {
+                    const float4 E = J.ment[k];
+                    const float4 X = J.lig2[(size_t)idx_lig * NL + __float_as_int(E.w)];
+                    const float ax = r00 * X.x + r01 * X.y + r02 * X.z + tx;
+                    const float ay = r10 * X.x + r11 * X.y + r12 * X.z + ty;
+                    const float az = r20 * X.x + r21 * X.y + r22 * X.z + tz;
+                    const float dx = ax - E.x, dy = ay - E.y, dz = az - E.z;
+                    s += fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                }
+                mcs += J.mtcc[row] * sqrtf(s / (float)(re - rb));
+            }
+            float *v = vals + (size_t)slot * 8;
+            v[0] = evdw;
+            v[1] = lc * eele;
+            v[2] = epmf;
+            v[3] = epsp;
+            v[4] = ehdb;
+            v[5] = hpc_atom;
+            v[6] = kde_atom;
+            v[7] = mcs;
+        }
+        __syncthreads();
+
+        // ---- per-replica sums: thread (g, k) adds term k over the atoms ----
+        for (int t = threadIdx.x; t < R * 8; t += blockDim.x) {
+            const int gg = t >> 3, k = t & 7;
+            float s = 0.f, c = 0.f;
+            const float *v = vals + (size_t)gg * NL * 8 + k;
+            for (int i = 0; i < NL; ++i) acc_add<KAHAN>(s, c, v[(size_t)i * 8]);
+            rsum[gg * 16 + k] = s;
+        }
+        __syncthreads();
+
+        // ---- normalise, total, accept (kernel_cpp_montecarlo_soa.cpp:333-390) ----
+        for (int gg = threadIdx.x; gg < R; gg += blockDim.x) {
+            float e[GD_MAXWEI];
+            const float *pm = prop + gg * 8;
+            for (int k = 0; k < 6; ++k) e[k] = rsum[gg * 16 + k];
+            e[GD_E_HDB] = e[GD_E_HDB] * (-1.0f / sqrtf(2.0f * GD_PI));
+            e[GD_E_KDE] = rsum[gg * 16 + 6] * sc.kde3inv;
+            e[GD_E_LHM] = logf(rsum[gg * 16 + 7] / (float)J.nrow);
+            float dst = 0.f;
+            for (int k = 0; k < 3; ++k) {
+                const float dd = J.center[k] + pm[k] - tg.pocket[k];
+                dst += dd * dd;
+            }
+            e[GD_E_DST] = sqrtf(dst);
+            for (int k = 0; k < 7; ++k) e[k] = e[k] / (float)NL;
+            for (int k = 0; k < GD_MAXWEI - 1; ++k) e[k] = sc.a_para[k] * e[k] + sc.b_para[k];
+            float etotal = 0.f;
+            for (int k = 0; k < GD_MAXWEI - 1; ++k) etotal += sc.w[k] * e[k];
+            e[GD_E_TOTAL] = etotal;
+
+            if (MODE == MODE_ENERGY) {
+                for (int k = 0; k < GD_MAXWEI; ++k) pose_out[(size_t)pose * GD_MAXWEI + k] = e[k];
+            } else {
+                const int stp = s1 + s2;
+                float *cur = st + gg * 16;
+                int acc;
+                if (stp != 0) {
+                    const float delta = etotal - cur[6 + GD_E_TOTAL];
+                    acc = pm[6] < expf(delta * J.mbeta[ist[gg * 4 + 3]]);
+                } else {
+                    acc = 1;
+                }
+                ist[gg * 4 + 1] = acc;
+                if (acc) {
+                    for (int k = 0; k < 6; ++k) cur[k] = pm[k];
+                    for (int k = 0; k < GD_MAXWEI; ++k) cur[6 + k] = e[k];
+                    ist[gg * 4 + 2] = stp;
+                    if (keep_records) {
+                        const int r = task.rep0 + gg;
+                        const int ne = ist[gg * 4 + 0];
+                        gd_ReplicaMC *rc = J.rec + (size_t)r * J.rec_cap + ne;
+                        const gd_ReplicaMC &src = J.rep[r];
+                        rc->idx_rep = src.idx_rep;
+                        rc->idx_lig = src.idx_lig;
+                        rc->idx_prt = src.idx_prt;
+                        rc->idx_tmp = ist[gg * 4 + 3];
+                        for (int k = 0; k < 6; ++k) rc->movematrix[k] = cur[k];
+                        for (int k = 0; k < GD_MAXWEI; ++k) rc->energy[k] = cur[6 + k];
+                        rc->step = stp;
+                        rc->is_accept = src.is_accept;
+                        ist[gg * 4 + 0] = ne + 1;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    if (MODE == MODE_MC) {
+        for (int i = threadIdx.x; i < R; i += blockDim.x) {
+            gd_ReplicaMC &rp = J.rep[task.rep0 + i];
+            for (int k = 0; k < 6; ++k) rp.movematrix[k] = st[i * 16 + k];
+            for (int k = 0; k < GD_MAXWEI; ++k) rp.energy[k] = st[i * 16 + 6 + k];
+            rp.step = ist[i * 4 + 2];
+            rp.is_accept = ist[i * 4 + 1];
+            if (keep_records) J.next_entry[task.rep0 + i] = ist[i * 4 + 0];
+        }
+    }
+}
+
+// One thread per ladder (ligand conformation l, protein conformation p):
+// temperature replica exchange, see gd_exchange in geauxdock_amd.h.
+__global__ void exchange_kernel(const DevJob *__restrict__ job, int n_ladder, int n_lig, int n_tmp, int round,
+                                const int *__restrict__ round_base, int *__restrict__ accept_count)
+{
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= n_ladder) return;
+    const int rnd = round + (round_base ? *round_base : 0);
+    const DevJob &J = *job;
+    const int p = L / n_lig, l = L - p * n_lig;
+    int holder[64];
+    for (int t = 0; t < n_tmp; ++t) {
+        const int r = n_tmp * n_lig * p + n_lig * t + l;
+        holder[J.rep[r].idx_tmp] = r;
+    }
+    int accepted = 0;
+    for (int t = rnd & 1; t + 1 < n_tmp; t += 2) {
+        const int ra = holder[t], rb = holder[t + 1];
+        const double beta_a = -(double)J.mbeta[t];
+        const double beta_b = -(double)J.mbeta[t + 1];
+        const double delta = (beta_a - beta_b) * ((double)J.rep[ra].energy[GD_E_TOTAL] - (double)J.rep[rb].energy[GD_E_TOTAL]);
+        const uint32_t j = (uint32_t)(t >> 1);
+        uint32_t c0 = (uint32_t)rnd, c1 = (uint32_t)L, c2 = 1u, c3 = j >> 2;
+        philox10(c0, c1, c2, c3, J.key0, J.key1);
+        const uint32_t wsel = (j & 3) == 0 ? c0 : ((j & 3) == 1 ? c1 : ((j & 3) == 2 ? c2 : c3));
+        const double u = (double)(wsel >> 1) * (1.0 / 2147483648.0);
+        if (u < exp(delta)) {
+            J.rep[ra].idx_tmp = t + 1;
+            J.rep[rb].idx_tmp = t;
+            ++accepted;
+        }
+    }
+    if (accept_count && accepted) atomicAdd(accept_count, accepted);
+}
+
+hipError_t launch_mc(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks, int ntask,
+                     int waves, int max_rep, bool compensated, bool keep_records, hipStream_t stream,
+                     const int *step_base)
+{
+    if (ntask <= 0) return hipSuccess;
+    const size_t lds = mc_lds_bytes(waves, max_rep);
+    const dim3 grid(ntask), block(waves * 64);
+    if (compensated)
+        hipLaunchKernelGGL((mc_kernel<true, MODE_MC>), grid, block, lds, stream, tg, sc, jobs, tasks, step_base,
+                           keep_records ? 1 : 0, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((mc_kernel<false, MODE_MC>), grid, block, lds, stream, tg, sc, jobs, tasks, step_base,
+                           keep_records ? 1 : 0, nullptr, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
+                         int ntask, int waves, bool compensated, const int *pose_rep, const float *pose_move,
+                         float *pose_out, hipStream_t stream)
+{
+    if (ntask <= 0) return hipSuccess;
+    const size_t lds = mc_lds_bytes(waves, 1);
+    const dim3 grid(ntask), block(waves * 64);
+    if (compensated)
+        hipLaunchKernelGGL((mc_kernel<true, MODE_ENERGY>), grid, block, lds, stream, tg, sc, jobs, tasks, nullptr, 0,
+                           pose_rep, pose_move, pose_out);
+    else
+        hipLaunchKernelGGL((mc_kernel<false, MODE_ENERGY>), grid, block, lds, stream, tg, sc, jobs, tasks, nullptr,
+                           0, pose_rep, pose_move, pose_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_exchange(const DevJob *job_dev, int n_ladder, int n_lig, int n_tmp, int round, int *accept_count,
+                           hipStream_t stream, const int *round_base)
+{
+    if (n_ladder <= 0 || n_tmp < 2) return hipSuccess;
+    const int bs = 64;
+    hipLaunchKernelGGL(exchange_kernel, dim3((n_ladder + bs - 1) / bs), dim3(bs), 0, stream, job_dev, n_ladder, n_lig,
+                       n_tmp, round, round_base, accept_count);
+    return hipGetLastError();
+}
+
+} // namespace gd

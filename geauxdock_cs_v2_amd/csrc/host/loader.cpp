@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <sstream>
 
 #include "codes.hpp"
@@ -263,9 +264,20 @@ bool load_protein(const std::string &path, RawJob &out, std::string &err)
     out.prt_y.assign((size_t)nconf * npoint, 0.0f);
     out.prt_z.assign((size_t)nconf * npoint, 0.0f);
 
+    // atoms of each residue serial, in file order: a point only ever takes
+    // atoms of its own residue (and, for a peptide plate, the previous one)
+    std::map<int, std::vector<int>> by_res;
+    for (int i = 0; i < (int)atoms.size(); ++i) by_res[atoms[i].resid].push_back(i);
+    const std::vector<int> none;
+    auto atoms_of = [&](int resid) -> const std::vector<int> & {
+        auto it = by_res.find(resid);
+        return it == by_res.end() ? none : it->second;
+    };
+
     // each point averages its atoms within each model; float sums in file
     // order (load.cpp:715-852)
     std::vector<float> sx(nconf), sy(nconf), sz(nconf), sn(nconf);
+    std::vector<int> cand;
     for (int p = 0; p < npoint; ++p) {
         std::fill(sx.begin(), sx.end(), 0.0f);
         std::fill(sy.begin(), sy.end(), 0.0f);
@@ -275,7 +287,16 @@ bool load_protein(const std::string &path, RawJob &out, std::string &err)
         const int resid = seq3[r];
         const int prev = (pt_t[p] == 1 && r > 0) ? seq3[r - 1] : -1;
         const PointAtomRule rule = point_atom_rule(pt_t[p]);
-        for (const Atom &a : atoms) {
+        const std::vector<int> &own = atoms_of(resid);
+        if (rule == RULE_PP && prev != resid) {
+            const std::vector<int> &pv = atoms_of(prev);
+            cand.resize(own.size() + pv.size());
+            std::merge(own.begin(), own.end(), pv.begin(), pv.end(), cand.begin());
+        } else {
+            cand = own;
+        }
+        for (int ai : cand) {
+            const Atom &a = atoms[ai];
             if (a.model >= nconf) continue;
             const std::string nm(a.name);
             bool take = false;

@@ -85,6 +85,8 @@ def _declare_device(lib):
     _sig(lib, "gd_read_replicas", i32, [vp, i32, C.POINTER(D.ReplicaMC)])
     _sig(lib, "gd_write_replicas", i32, [vp, i32, C.POINTER(D.ReplicaMC)])
     _sig(lib, "gd_read_records", i32, [vp, i32, C.POINTER(D.ReplicaMC), pi, i32])
+    if not hasattr(lib, "gd_batch_run"):
+        raise ImportError("libgeauxdock_amd.so lacks the batched and serving entry points: rebuild it")
     _sig(lib, "gd_batch_run", i32, [vp, pi, i32, C.POINTER(D.BatchOpts), C.POINTER(C.c_double)])
     _sig(lib, "gd_server_create", i32, [vp, i32, C.POINTER(D.ServerOpts), pp])
     _sig(lib, "gd_server_destroy", None, [vp])

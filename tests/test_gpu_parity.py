@@ -1,0 +1,244 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: energies within REL_TOL of the FP64 oracle (floor TERM_FLOOR, see
+conftest); the move proposals bit-exact (same stream, same float rounding);
+accept and exchange decisions identical except where the accept uniform sits
+within the energy tolerance of the threshold."""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import KAT_1A07C1, REL_TOL, TERM_FLOOR, rel_err
+from geauxdock_cs_v2_amd import ctypes_defs as D
+from geauxdock_cs_v2_amd import lib as L
+from oracle import bind as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx(lib):
+    h = C.c_void_p()
+    L.check(lib.gd_create(0, C.byref(h)))
+    yield h
+    lib.gd_destroy(h)
+
+
+def upload(lib, ctx, cx, seed=1234):
+    t, j = C.c_int(), C.c_int()
+    L.check(lib.gd_upload_target(ctx, C.byref(cx.view), C.byref(t)))
+    L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), seed, C.byref(j)))
+    return t.value, j.value
+
+
+def free(lib, ctx, t, j):
+    L.check(lib.gd_free_job(ctx, j))
+    L.check(lib.gd_free_target(ctx, t))
+
+
+def gpu_energy(lib, ctx, job, rep_idx, move):
+    rep_idx = np.ascontiguousarray(rep_idx, np.int32)
+    move = np.ascontiguousarray(move, np.float32).reshape(-1, 6)
+    out = np.zeros((len(rep_idx), 10), np.float32)
+    L.check(lib.gd_energy(ctx, job, len(rep_idx), rep_idx.ctypes.data_as(C.POINTER(C.c_int)),
+                          move.ctypes.data_as(C.POINTER(C.c_float)), out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out
+
+
+def read_reps(lib, ctx, job, n):
+    arr = (D.ReplicaMC * n)()
+    L.check(lib.gd_read_replicas(ctx, job, arr))
+    return arr
+
+
+def test_kat_on_gpu(lib, ctx, kat_complex):
+    say("case: KAT 1a07C1 step-0 energy on the GPU")
+    _, cx = kat_complex
+    t, j = upload(lib, ctx, cx)
+    e = gpu_energy(lib, ctx, j, [0], np.zeros(6))[0]
+    free(lib, ctx, t, j)
+    assert np.all(np.abs(e - KAT_1A07C1) <= 5e-5), e
+
+
+@pytest.mark.parametrize("which", ["kat11", "c1", "c2"])
+def test_pose_energies_vs_fp64(lib, ctx, kat_complex11, which):
+    from geauxdock_cs_v2_amd import synth
+    if which == "kat11":
+        cx = kat_complex11[1]
+    else:
+        cx, _ = synth.load_config(which)
+    n_rep = cx.size["n_rep"]
+    rng = np.random.default_rng(7)
+    n = min(256, n_rep)
+    idx = rng.choice(n_rep, n, replace=False).astype(np.int32)
+    move = np.concatenate([rng.normal(0, 0.6, (n, 3)), rng.uniform(-math.pi, math.pi, (n, 3))], 1).astype(np.float32)
+    move[0] = 0
+    say(f"case: pose energies {which} n={n}")
+    t, j = upload(lib, ctx, cx)
+    eg = gpu_energy(lib, ctx, j, idx, move)
+    free(lib, ctx, t, j)
+    e64 = O.energy(cx.view, idx, move, "f64")
+    e32 = O.energy(cx.view, idx, move, "f32")
+    err_gpu = rel_err(eg, e64).max(0)
+    err_ref32 = rel_err(e32, e64).max(0)
+    say("max rel err vs FP64, GPU  :", np.array2string(err_gpu, precision=2))
+    say("max rel err vs FP64, CPU32:", np.array2string(err_ref32, precision=2))
+    assert np.all(np.isfinite(eg))
+    assert err_gpu.max() <= REL_TOL, err_gpu
+
+
+def test_montecarlo_chunks_match_oracle(lib, ctx):
+    """Plain chunks of the C1 workload step by step: after every step the GPU
+    state is written into the oracle's copy (resynchronised), the oracle takes
+    the same step, and decisions, moves and energies are compared."""
+    from geauxdock_cs_v2_amd import synth
+    cx, _ = synth.load_config("c1")
+    n = cx.size["n_rep"]
+    seed = 31337
+    t, j = upload(lib, ctx, cx, seed)
+    flips = 0
+    for step in range(0, 60):
+        say(f"case: c1 plain step {step}")
+        before = read_reps(lib, ctx, j, n)
+        L.check(lib.gd_montecarlo(ctx, j, step, 1))
+        g = read_reps(lib, ctx, j, n)
+        o = (D.ReplicaMC * n)()
+        C.memmove(o, before, C.sizeof(o))
+        O.montecarlo(cx.view, o, step, 1, seed)
+        for r in range(n):
+            if g[r].is_accept != o[r].is_accept:
+                # allowed only at a tie: accept uniform within tolerance of the threshold
+                u = lib.gd_stream_k(seed, 0, g[r].idx_rep, step, 6) / 2147483648.0
+                mb = cx.view.temp[g[r].idx_tmp].minus_beta
+                e_new = o[r].energy[9] if o[r].is_accept else g[r].energy[9]
+                thr = math.exp((e_new - before[r].energy[9]) * mb)
+                assert abs(u - thr) <= 1e-4 * max(1.0, abs(thr)), (step, r, u, thr)
+                flips += 1
+                continue
+            assert np.array_equal(np.array(g[r].movematrix[:6]), np.array(o[r].movematrix[:6])), (step, r)
+            e_g, e_o = np.array(g[r].energy[:10]), np.array(o[r].energy[:10], np.float64)
+            assert rel_err(e_g, e_o).max() <= 3 * REL_TOL, (step, r, e_g, e_o)
+            assert g[r].step == o[r].step
+    free(lib, ctx, t, j)
+    say("near-tie decision flips:", flips)
+    assert flips <= 2
+
+
+def test_montecarlo_long_chunk_records(lib, ctx, kat_complex11):
+    """A 100-step chunk of the 11-conformation 1a07C job: the record window
+    holds exactly the accepted steps, in order, ending at the final state."""
+    cx = kat_complex11[1]
+    n = cx.size["n_rep"]
+    say("case: kat11 plain 100-step chunk records")
+    t, j = upload(lib, ctx, cx, 5)
+    L.check(lib.gd_montecarlo(ctx, j, 0, 100))
+    g = read_reps(lib, ctx, j, n)
+    rec = (D.ReplicaMC * (n * 100))()
+    ne = np.zeros(n, np.int32)
+    L.check(lib.gd_read_records(ctx, j, rec, ne.ctypes.data_as(C.POINTER(C.c_int)), 100))
+    free(lib, ctx, t, j)
+    assert np.all(ne >= 1) and np.all(ne <= 100)
+    for r in range(0, n, 97):
+        steps = [rec[r * 100 + k].step for k in range(ne[r])]
+        assert steps[0] == 0 and steps == sorted(steps)
+        last = rec[r * 100 + ne[r] - 1]
+        assert last.step == g[r].step and list(last.movematrix) == list(g[r].movematrix)
+
+
+def test_exchange_matches_oracle(lib, ctx, kat_complex11):
+    cx = kat_complex11[1]
+    n = cx.size["n_rep"]
+    seed = 99
+    t, j = upload(lib, ctx, cx, seed)
+    rng = np.random.default_rng(3)
+    reps = cx.replicas()
+    for r in range(n):
+        reps[r].energy[9] = float(rng.normal(0, 0.05))
+    L.check(lib.gd_write_replicas(ctx, j, reps))
+    for rnd in range(6):
+        say(f"case: exchange round {rnd}")
+        acc = C.c_int()
+        L.check(lib.gd_exchange(ctx, j, rnd, C.byref(acc)))
+        want = O.exchange(cx.view, reps, rnd, seed)
+        g = read_reps(lib, ctx, j, n)
+        assert acc.value == want
+        assert [g[r].idx_tmp for r in range(n)] == [reps[r].idx_tmp for r in range(n)]
+    free(lib, ctx, t, j)
+
+
+def _batch(lib, ctx, jobs, steps, exchange=0, graph=0, comp=0, waves=0, records=0):
+    o = D.BatchOpts(steps, exchange, graph, 0, waves, 0, comp, records)
+    arr = (C.c_int * len(jobs))(*jobs)
+    ms = C.c_double()
+    L.check(lib.gd_batch_run(ctx, arr, len(jobs), C.byref(o), C.byref(ms)))
+    return ms.value
+
+
+def test_batched_steps_match_oracle(lib, ctx):
+    """Batched form (packed replicas, autotuned shape) step by step against
+    the oracle, resynchronised as in the plain test."""
+    from geauxdock_cs_v2_amd import synth
+    cx, _ = synth.load_config("c1")
+    n = cx.size["n_rep"]
+    seed = 8
+    t, j = upload(lib, ctx, cx, seed)
+    flips = 0
+    for step in range(0, 30):
+        say(f"case: c1 batched step {step}")
+        before = read_reps(lib, ctx, j, n)
+        _batch(lib, ctx, [j], 1)
+        g = read_reps(lib, ctx, j, n)
+        o = (D.ReplicaMC * n)()
+        C.memmove(o, before, C.sizeof(o))
+        O.montecarlo(cx.view, o, step, 1, seed)
+        for r in range(n):
+            if g[r].is_accept != o[r].is_accept:
+                flips += 1
+                continue
+            assert np.array_equal(np.array(g[r].movematrix[:6]), np.array(o[r].movematrix[:6]))
+            assert rel_err(np.array(g[r].energy[:10]), np.array(o[r].energy[:10], np.float64)).max() <= 3 * REL_TOL
+    free(lib, ctx, t, j)
+    assert flips <= 2
+
+
+def test_graph_replay_is_bit_identical(lib, ctx):
+    """The same batched run with and without hipGraph capture, with exchange."""
+    from geauxdock_cs_v2_amd import synth
+    cx, _ = synth.load_config("c1")
+    n = cx.size["n_rep"]
+    res = []
+    for graph in (0, 1):
+        say(f"case: c1 batched 300 steps exchange 100 graph={graph}")
+        t, j = upload(lib, ctx, cx, 77)
+        _batch(lib, ctx, [j], 300, exchange=100, graph=graph, waves=8)
+        res.append(bytes(read_reps(lib, ctx, j, n)))
+        free(lib, ctx, t, j)
+    assert res[0] == res[1]
+
+
+def test_compensated_accumulation_within_tolerance(lib, ctx):
+    """Kahan-compensated FP32 accumulation: 20 steps, final energies
+    re-evaluated by the FP64 oracle at the GPU's final moves."""
+    from geauxdock_cs_v2_amd import synth
+    cx, _ = synth.load_config("c2")
+    n = cx.size["n_rep"]
+    say("case: c2 batched compensated 20 steps")
+    t, j = upload(lib, ctx, cx, 21)
+    _batch(lib, ctx, [j], 20, comp=1)
+    g = read_reps(lib, ctx, j, n)
+    free(lib, ctx, t, j)
+    idx = np.arange(0, n, 37, dtype=np.int32)
+    mm = np.array([list(g[i].movematrix)[:6] for i in idx], np.float32)
+    e64 = O.energy(cx.view, idx, mm, "f64")
+    eg = np.array([list(g[i].energy)[:10] for i in idx])
+    err = rel_err(eg, e64).max(0)
+    say("compensated max rel err vs FP64:", np.array2string(err, precision=2))
+    assert err.max() <= REL_TOL
