@@ -437,7 +437,7 @@ int gd_montecarlo(gd_ctx *ctx, int job_id, int s1, int s2max)
     std::vector<DevTask> tasks(j->n_rep);
     for (int r = 0; r < j->n_rep; ++r)
         tasks[r] = DevTask{0, r, 1, j->rep_prt[r], s1, s2max, sh.slices, sh.slot_waves};
-    if (int rc = run_tasks(ctx, t, j->self.as<DevJob>(), tasks, sh.waves, 1, false, true)) return rc;
+    if (int rc = run_tasks(ctx, t, j->self.as<DevJob>(), tasks, sh.waves, 1, true, true)) return rc;
     GD_HIP(hipStreamSynchronize(ctx->stream));
     return GD_OK;
 }
@@ -477,7 +477,7 @@ int gd_energy(gd_ctx *ctx, int job_id, int n, const int *rep_idx, const float *m
     GD_HIP(upload(ctx->pose_rep, rep_idx, sizeof(int) * n, s));
     GD_HIP(upload(ctx->pose_move, move, sizeof(float) * 6 * n, s));
     if (ctx->pose_out.bytes < sizeof(float) * GD_MAXWEI * n) GD_HIP(ctx->pose_out.alloc(sizeof(float) * GD_MAXWEI * n));
-    GD_HIP(launch_energy(t.dt, t.sc, j->self.as<DevJob>(), ctx->task_buf.as<DevTask>(), n, sh.waves, false,
+    GD_HIP(launch_energy(t.dt, t.sc, j->self.as<DevJob>(), ctx->task_buf.as<DevTask>(), n, sh.waves, true,
                          ctx->pose_rep.as<int>(), ctx->pose_move.as<float>(), ctx->pose_out.as<float>(), s));
     GD_HIP(hipMemcpyAsync(e_out, ctx->pose_out.p, sizeof(float) * GD_MAXWEI * n, hipMemcpyDeviceToHost, s));
     GD_HIP(hipStreamSynchronize(s));

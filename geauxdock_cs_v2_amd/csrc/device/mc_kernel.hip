@@ -67,6 +67,8 @@ struct Slot {
     int tl;        // ligand type
 };
 
+constexpr int kBlock = 64; // points per plain FP32 partial sum
+
 } // namespace
 
 size_t mc_lds_bytes(int waves, int nrep)
@@ -198,8 +200,13 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
             const float p1a = pA.x, p2a = pA.y, pmf0 = pA.z, pmf1 = pA.w, hdb0 = pB.x, hdb1 = pB.y;
+            // plain FP32 sums over blocks of at most kBlock points, block sums
+            // added with compensation: the electrostatic sum cancels by up to
+            // 1e4 over a pose, which long running FP32 sums cannot hold
+            for (int pb = b; pb < e; pb += kBlock) {
+            const int pe = min(pb + kBlock, e);
             float sv = 0.f, se = 0.f, sp = 0.f, ss = 0.f, sh = 0.f, sq = 0.f;
-            for (int p = b; p < e; ++p) {
+            for (int p = pb; p < pe; ++p) {
                 const float4 P4 = ptc[p];
                 const float2 A = tg.pa[p];
                 const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
@@ -246,6 +253,7 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
             acc_add<KAHAN>(epsp, cpsp, ss);
             acc_add<KAHAN>(ehdb, chdb, sh);
             acc_add<KAHAN>(ehpc, chpc, sq);
+            }
         }
 
         // ---- cross-slice sum ----

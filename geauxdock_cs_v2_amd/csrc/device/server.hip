@@ -193,7 +193,7 @@ void gd_server::loop()
         if (e == hipSuccess) e = upload(tasks_dev, tasks.data(), sizeof(DevTask) * tasks.size(), ctx->stream);
         if (e == hipSuccess)
             e = launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), tasks_dev.as<DevTask>(), (int)tasks.size(), kServerWaves,
-                          max_rep, false, false, ctx->stream, nullptr);
+                          max_rep, true, false, ctx->stream, nullptr);
         std::vector<int> exch(ids.size(), 0);
         if (e == hipSuccess && opts.exchange_interval > 0) {
             std::lock_guard<std::mutex> lk(mu);

@@ -80,7 +80,7 @@ class Device:
         L.check(self.lib.gd_read_records(self.h, job, rec, ne.ctypes.data_as(C.POINTER(C.c_int)), cap))
         return rec, ne
 
-    def batch_run(self, jobs, steps, exchange=0, graph=True, compensated=False, records=False, waves=0,
+    def batch_run(self, jobs, steps, exchange=0, graph=True, compensated=True, records=False, waves=0,
                   slots_per_wave=0, point_slices=0):
         """Advance every job by `steps` MC steps in fused launches; returns the
         device time in ms (hipEvents around the chunk sequence)."""

@@ -1,0 +1,130 @@
+"""GPU tests of the serving loop (continuous batching, HBM-footprint
+admission): a request's result is the same alone and in a mix, requests that
+do not fit wait for room, cancelled and malformed requests fail alone."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import REL_TOL, rel_err
+from geauxdock_cs_v2_amd import ctypes_defs as D
+from geauxdock_cs_v2_amd import lib as L
+from geauxdock_cs_v2_amd import synth
+from oracle import bind as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def setup(lib, tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("serve"))
+    base = synth.generate(d, "sA", seed=1, n_atoms=20, n_lig_conf=3, lig_sigma=0.4, n_points=800, n_prt_conf=2,
+                          prt_sigma=0.4, prt_radius=18.0, n_kde=60, mcs_rows=4, mcs_cols=8)
+    jobs = {"A": base}
+    for name, seed, na, nc in [("B", 2, 37, 5), ("C", 3, 11, 2), ("D", 4, 70, 2)]:
+        jobs[name] = synth.generate(d, "s" + name, seed=seed, n_atoms=na, n_lig_conf=nc, lig_sigma=0.4,
+                                    n_points=800, n_prt_conf=2, prt_sigma=0.4, prt_radius=18.0, n_kde=60,
+                                    mcs_rows=4, mcs_cols=8, protein=base["protein"])
+    cx = {}
+    for k, v in jobs.items():
+        raw = L.load_job(v["pdb"], v["sdf"], v["ff"], "s" + k, n_tmp=4, temp_high=2.0, temp_low=0.3)
+        cx[k] = raw.stage()
+    h = C.c_void_p()
+    L.check(lib.gd_create(0, C.byref(h)))
+    t = C.c_int()
+    L.check(lib.gd_upload_target(h, C.byref(cx["A"].view), C.byref(t)))
+    yield lib, h, t.value, cx
+    lib.gd_free_target(h, t.value)
+    lib.gd_destroy(h)
+
+
+def serve(lib, h, t, reqs, budget=1 << 34, max_active=16, chunk=50, exchange=50):
+    opts = D.ServerOpts(budget, max_active, chunk, exchange)
+    srv = C.c_void_p()
+    L.check(lib.gd_server_create(h, t, C.byref(opts), C.byref(srv)))
+    ids = []
+    for cx, steps, seed in reqs:
+        i = C.c_int()
+        L.check(lib.gd_server_submit(srv, C.byref(cx.view), steps, seed, C.byref(i)))
+        ids.append(i.value)
+    L.check(lib.gd_server_drain(srv))
+    out = []
+    for (cx, _, _), i in zip(reqs, ids):
+        res = D.RequestResult()
+        reps = (D.ReplicaMC * cx.size["n_rep"])()
+        L.check(lib.gd_server_poll(srv, i, C.byref(res), reps))
+        out.append((res.status, bytes(reps), res, reps))
+    lib.gd_server_destroy(srv)
+    return out
+
+
+def test_alone_and_in_the_mix_are_identical(setup):
+    lib, h, t, cx = setup
+    say("case: serving A alone")
+    alone = serve(lib, h, t, [(cx["A"], 300, 11)])
+    say("case: serving A mixed with B, C, D")
+    mixed = serve(lib, h, t, [(cx["B"], 200, 12), (cx["A"], 300, 11), (cx["C"], 450, 13), (cx["D"], 100, 14)])
+    assert alone[0][0] == 0 and all(m[0] == 0 for m in mixed)
+    assert alone[0][1] == mixed[1][1]
+
+
+def test_served_energies_match_fp64(setup):
+    lib, h, t, cx = setup
+    say("case: serving B, energies vs FP64")
+    (st, _, _, reps), = serve(lib, h, t, [(cx["B"], 120, 5)])
+    assert st == 0
+    c = cx["B"]
+    n = c.size["n_rep"]
+    idx = np.arange(n, dtype=np.int32)
+    mm = np.array([list(r.movematrix)[:6] for r in reps], np.float32)
+    e64 = O.energy(c.view, idx, mm, "f64")
+    eg = np.array([list(r.energy)[:10] for r in reps])
+    assert rel_err(eg, e64).max() <= REL_TOL
+    assert all(0 <= r.step < 120 for r in reps)
+
+
+def test_admission_by_footprint(setup):
+    """A budget that holds one request at a time serialises the requests
+    (each admitted after the previous finished) without changing results."""
+    lib, h, t, cx = setup
+    fp = lib.gd_job_footprint(cx["B"].size["lig_natom"], cx["B"].size["n_lig"], cx["B"].size["kde_npoint"],
+                              int(cx["B"].array("mcs_rowptr")[-1]), cx["B"].size["mcs_nrow"],
+                              cx["B"].size["n_tmp"], cx["B"].size["n_rep"], 0)
+    say("case: serving with a one-job budget")
+    tight = serve(lib, h, t, [(cx["B"], 100, 1), (cx["B"], 100, 2), (cx["B"], 100, 3)], budget=int(fp * 1.5))
+    loose = serve(lib, h, t, [(cx["B"], 100, 1), (cx["B"], 100, 2), (cx["B"], 100, 3)])
+    assert [x[1] for x in tight] == [x[1] for x in loose]
+    r = [x[2] for x in tight]
+    assert r[1].t_admit_ms >= r[0].t_done_ms - 1e-6 and r[2].t_admit_ms >= r[1].t_done_ms - 1e-6
+
+
+def test_cancel_and_bad_requests_fail_alone(setup):
+    lib, h, t, cx = setup
+    opts = D.ServerOpts(1 << 34, 1, 50, 0)
+    srv = C.c_void_p()
+    L.check(lib.gd_server_create(h, t, C.byref(opts), C.byref(srv)))
+    a, b = C.c_int(), C.c_int()
+    say("case: serving cancel + rejected request")
+    L.check(lib.gd_server_submit(srv, C.byref(cx["A"].view), 200, 1, C.byref(a)))
+    L.check(lib.gd_server_submit(srv, C.byref(cx["C"].view), 200, 2, C.byref(b)))
+    L.check(lib.gd_server_cancel(srv, b.value))
+    # a request larger than the whole budget is refused at submission
+    small = D.ServerOpts(1024, 1, 50, 0)
+    srv2 = C.c_void_p()
+    L.check(lib.gd_server_create(h, t, C.byref(small), C.byref(srv2)))
+    c = C.c_int()
+    assert lib.gd_server_submit(srv2, C.byref(cx["A"].view), 10, 1, C.byref(c)) == L.GD_ERR_NOMEM
+    lib.gd_server_destroy(srv2)
+    L.check(lib.gd_server_drain(srv))
+    ra, rb = D.RequestResult(), D.RequestResult()
+    L.check(lib.gd_server_poll(srv, a.value, C.byref(ra), None))
+    L.check(lib.gd_server_poll(srv, b.value, C.byref(rb), None))
+    lib.gd_server_destroy(srv)
+    assert ra.status == 0
+    assert rb.status == L.GD_ERR_STATE
