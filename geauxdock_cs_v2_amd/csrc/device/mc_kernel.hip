@@ -49,6 +49,15 @@ __device__ __forceinline__ float uniform31(uint32_t w) { return (float)(w >> 1) 
 
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
+// scale * u + m rounded twice, as the reference computes the proposal
+// (kernel_cpp_montecarlo_soa.cpp:234); contraction into an FMA would change
+// the last bit of the cumulative move
+__device__ __forceinline__ float move_update(float scale, float u, float m)
+{
+#pragma clang fp contract(off)
+    return scale * u + m;
+}
+
 template <bool KAHAN>
 __device__ __forceinline__ void acc_add(float &s, float &c, float v)
 {
@@ -161,8 +170,7 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
                 philox10(c0, c1, c2, c3, J.key0, J.key1);
                 const float u[7] = {uniform31(b0), uniform31(b1), uniform31(b2), uniform31(b3),
                                     uniform31(c0), uniform31(c1), uniform31(c2)};
-                // same rounding as the reference's scale * u + m (no contraction)
-                for (int k = 0; k < 6; ++k) m[k] = __fadd_rn(__fmul_rn(J.scale[k], u[k]), m[k]);
+                for (int k = 0; k < 6; ++k) m[k] = move_update(J.scale[k], u[k], m[k]);
                 uacc = u[6];
             }
         } else {
