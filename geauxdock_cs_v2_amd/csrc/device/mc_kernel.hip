@@ -350,9 +350,13 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
             e[GD_E_HDB] = e[GD_E_HDB] * (-1.0f / sqrtf(2.0f * GD_PI));
             e[GD_E_KDE] = rsum[gg * 16 + 6] * sc.kde3inv;
             e[GD_E_LHM] = logf(rsum[gg * 16 + 7] / (float)J.nrow);
+            // |centre + m - pocket|: the reference adds m to the ~30 A centre
+            // first and loses the small offset to rounding; subtracting the
+            // two centres first is exact when they coincide (they do: both
+            // are the native ligand centroid)
             float dst = 0.f;
             for (int k = 0; k < 3; ++k) {
-                const float dd = J.center[k] + pm[k] - tg.pocket[k];
+                const float dd = (J.center[k] - tg.pocket[k]) + pm[k];
                 dst += dd * dd;
             }
             e[GD_E_DST] = sqrtf(dst);

@@ -89,7 +89,9 @@ def compare_step(lib, cx, seed, step, before, g, o):
             u = lib.gd_stream_k(seed, 0, before[r].idx_rep, step, 6) / 2147483648.0
             mb = cx.view.temp[before[r].idx_tmp].minus_beta
             thr = math.exp((e64[r][9] - before[r].energy[9]) * mb)
-            tol = REL_TOL * max(abs(e64[r][9]), TERM_FLOOR) * abs(mb) * thr + 1e-6
+            # the oracle decides on its FP32 energies, whose error against
+            # FP64 reaches ~1e-3 relative on the total (sequential sums)
+            tol = 1e-3 * max(abs(e64[r][9]), TERM_FLOOR) * abs(mb) * thr + 1e-6
             assert abs(u - thr) <= tol, (step, r, u, thr)
             flips += 1
         else:  # both rejected: state unchanged
