@@ -165,41 +165,49 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     if (np <= 0 || nconf <= 0) return fail(GD_ERR_ARG, "empty protein");
     const gd_EnePara &ep = cx->enepara;
 
-    // points re-sorted by type, keeping the class order within a type
-    std::vector<int> ord(np);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cx->prt_t[x] < cx->prt_t[y]; });
     for (int i = 0; i < np; ++i)
         if (cx->prt_t[i] < 0 || cx->prt_t[i] >= GD_MAXTP1) return fail(GD_ERR_ARG, "protein point type out of range");
 
-    // PSP rows of residues that carry at least one entry, for class-2 points
+    // per point: hydrophobic-contact weight (cdc ? hpp : 0) and the PSP row
+    // of its residue (class-2 points of residues that carry PSP entries)
     std::map<int, int> slot_of;
     std::vector<float> pspc;
-    std::vector<float2> pa(np);
-    for (int i = 0; i < np; ++i) {
-        const int s = ord[i];
-        float2 v;
-        v.x = cx->prt_cdc[s] == 1 ? cx->prt_hpp[s] : 0.0f;
-        int slot = -1;
+    std::vector<float> hpcw(np);
+    std::vector<int> slot(np, -1), flags(np);
+    for (int s = 0; s < np; ++s) {
+        hpcw[s] = cx->prt_cdc[s] == 1 ? cx->prt_hpp[s] : 0.0f;
         if (cx->prt_c[s] == 2) {
             const int res = cx->prt_seq3r[s];
             if (res < 0 || res >= cx->psp_width) return fail(GD_ERR_ARG, "residue serial outside the PSP table");
             auto it = slot_of.find(res);
             if (it != slot_of.end()) {
-                slot = it->second;
+                slot[s] = it->second;
             } else {
                 bool any = false;
                 for (int t = 0; t < GD_MAXTP2; ++t) any |= cx->psp[(size_t)t * cx->psp_width + res] != 0.0f;
                 if (any) {
-                    slot = (int)slot_of.size();
-                    slot_of[res] = slot;
+                    slot[s] = (int)slot_of.size();
+                    slot_of[res] = slot[s];
                     for (int t = 0; t < GD_MAXTP2; ++t) pspc.push_back(cx->psp[(size_t)t * cx->psp_width + res]);
                 }
             }
         }
-        int bits;
-        std::memcpy(&bits, &slot, 4);
-        std::memcpy(&v.y, &bits, 4);
+        flags[s] = (hpcw[s] != 0.0f ? 1 : 0) | (slot[s] >= 0 ? 2 : 0);
+    }
+    // points re-sorted by (type, flags), keeping the class order otherwise:
+    // a segment shares one set of type-pair parameters and one term set
+    std::vector<int> ord(np);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        if (cx->prt_t[x] != cx->prt_t[y]) return cx->prt_t[x] < cx->prt_t[y];
+        return flags[x] > flags[y];
+    });
+    std::vector<float2> pa(np);
+    for (int i = 0; i < np; ++i) {
+        const int s = ord[i];
+        float2 v;
+        v.x = hpcw[s];
+        std::memcpy(&v.y, &slot[s], 4);
         pa[i] = v;
     }
     std::vector<float4> pt((size_t)nconf * np);
@@ -210,12 +218,13 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         }
     std::vector<int4> seg;
     for (int i = 0; i < np;) {
-        const int t = cx->prt_t[ord[i]];
+        const int t = cx->prt_t[ord[i]], f = flags[ord[i]];
         int j = i;
-        while (j < np && cx->prt_t[ord[j]] == t) ++j;
-        seg.push_back(make_int4(t, i, j, 0));
+        while (j < np && cx->prt_t[ord[j]] == t && flags[ord[j]] == f) ++j;
+        seg.push_back(make_int4(t, i, j, f));
         i = j;
     }
+    const float hdb_scale = std::sqrt(0.5f * 1.4426950408889634f); // folds exp(-x^2/2) into exp2
     std::vector<float> pp((size_t)GD_MAXTP1 * GD_MAXTP2 * PP_N, 0.0f);
     for (int tp = 0; tp < GD_MAXTP1; ++tp)
         for (int tl = 0; tl < GD_MAXTP2; ++tl) {
@@ -226,6 +235,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             q[PP_PMF1] = ep.pmf1[tl][tp];
             q[PP_HDB0] = ep.hdb0[tl][tp];
             q[PP_HDB1] = ep.hdb0[tl][tp] > 0.1f ? ep.hdb1[tl][tp] : 0.0f;
+            q[PP_HDB1S] = q[PP_HDB1] * hdb_scale;
         }
     if (pspc.empty()) pspc.assign(GD_MAXTP2, 0.0f);
 

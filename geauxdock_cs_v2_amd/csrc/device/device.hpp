@@ -7,7 +7,8 @@
 //                                     type) so a run of points shares one set
 //                                     of type-pair parameters
 //           pa   float2[npad]         {hpc weight = cdc ? hpp : 0, PSP slot}
-//           seg  int4[nseg]           {point type, begin, end, 0}
+//           seg  int4[nseg]           {point type, begin, end, flags}: flags bit 0 =
+//                                     hydrophobic points, bit 1 = PSP points
 //           pp   float[31][24][8]     type-pair parameters, see PairParam
 //           pspc float[nslot][24]     PSP rows of residues that have entries
 //   job     lig3 float4[nlig][natom]  {x3, y3, z3, c3} type-sorted atoms
@@ -34,6 +35,7 @@ enum PairParam {
     PP_PMF1 = 3, // contact depth
     PP_HDB0 = 4, // H-bond distance
     PP_HDB1 = 5, // H-bond inverse width, 0 where the pair has no H-bond term
+    PP_HDB1S = 6, // PP_HDB1 * sqrt(log2(e) / 2)
     PP_N = 8
 };
 

@@ -80,6 +80,73 @@ constexpr int kBlock = 64; // points per plain FP32 partial sum
 
 } // namespace
 
+namespace {
+
+struct PairParams {
+    float p1a, p2a, pmf0, pmf1, hdb0, hdb1, hdb1s;
+};
+
+struct Sums {
+    float v, e, p, s, h, q; // vdw, ele (without the atom charge), pmf, psp, hdb, hydrophobic
+};
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+// Six pair terms of one atom against points [pb, pe) of one type segment
+// (src/backend_cpu_mic/energy_prt_v1.cpp:30-146).  HPC: every point of the
+// segment is a hydrophobic-contact point (cdc with nonzero hpp); PSP: every
+// point carries a PSP row.  Points without them live in other segments, so
+// the loop needs no per-point branch for either term.
+template <bool HPC, bool PSP>
+__device__ __forceinline__ void pair_block(const float4 *__restrict__ ptc, const float2 *__restrict__ pa,
+                                           const float *__restrict__ pspc, int pb, int pe, const Slot &me,
+                                           const PairParams &q, const DevScalars &sc, Sums &s)
+{
+#pragma unroll 2
+    for (int p = pb; p < pe; ++p) {
+        const float4 P4 = ptc[p];
+        const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float rinv = __builtin_amdgcn_rsqf(d2);
+        const float d = d2 * rinv;
+        // hydrophobic contact inside 9 A (:44-50)
+        if (HPC) {
+            const float u3 = fmaf(-d2, 0.5f / 81.0f / 81.0f / 81.0f / 81.0f, 2.5f / 81.0f / 81.0f / 81.0f);
+            const float u2 = fmaf(-d2, u3, 4.5f / 81.0f / 81.0f);
+            const float u1 = fmaf(-d2, u2, 3.5f / 81.0f);
+            const float poly = fmaf(-d2, u1, 1.0f);
+            s.q = fmaf(pa[p].x, d2 <= 81.0f ? poly : 0.0f, s.q);
+        }
+        // soft-core Lennard-Jones (:62-71)
+        const float r2 = rinv * rinv;
+        const float r3 = r2 * rinv;
+        const float r6 = r3 * r3;
+        const float pp1 = q.p1a * (r6 * r3);
+        const float pp2 = q.p2a * r6;
+        const float pp4 = fmaf(pp1 * sc.lj0, fmaf(sc.lj1, d2, 1.0f), 1.0f);
+        s.v = fmaf(pp1 - pp2, __builtin_amdgcn_rcpf(pp4), s.v);
+        // electrostatics with its cubic core (:79-85)
+        const float s1e = sc.el1 * d;
+        const float gpoly = fmaf(s1e * s1e, fmaf(sc.b1, s1e, sc.a1), sc.el0);
+        const float g1 = s1e < 1.0f ? gpoly : rinv * sc.inv_el1;
+        s.e = fmaf(P4.w, g1, s.e);
+        // contact potential pmf1 / (1 + exp((6 - d/2)(d - pmf0))) (:95-98)
+        const float dm = d - q.pmf0;
+        const float xe = fmaf(-0.5f * kLog2e, d, 6.0f * kLog2e) * dm;
+        s.p = fmaf(q.pmf1, __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(xe)), s.p);
+        // pocket-specific term where d <= pmf0 (:107-109)
+        if (PSP) {
+            if (dm <= 0.0f) s.s += pspc[__float_as_int(pa[p].y) * GD_MAXTP2 + me.tl];
+        }
+        // hydrogen bond hdb1 exp(-((d - hdb0) hdb1)^2 / 2), hdb1 == 0 where
+        // the pair has none (:138-143); hdb1s = hdb1 sqrt(log2(e) / 2)
+        const float h3 = (d - q.hdb0) * q.hdb1s;
+        s.h = fmaf(q.hdb1, __builtin_amdgcn_exp2f(-h3 * h3), s.h);
+    }
+}
+
+} // namespace
+
 size_t mc_lds_bytes(int waves, int nrep)
 {
     const int lanes = waves * 64;
@@ -207,60 +274,25 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
-            const float p1a = pA.x, p2a = pA.y, pmf0 = pA.z, pmf1 = pA.w, hdb0 = pB.x, hdb1 = pB.y;
+            const PairParams q{pA.x, pA.y, pA.z, pA.w, pB.x, pB.y, pB.z};
             // plain FP32 sums over blocks of at most kBlock points, block sums
             // added with compensation: the electrostatic sum cancels by up to
             // 1e4 over a pose, which long running FP32 sums cannot hold
             for (int pb = b; pb < e; pb += kBlock) {
-            const int pe = min(pb + kBlock, e);
-            float sv = 0.f, se = 0.f, sp = 0.f, ss = 0.f, sh = 0.f, sq = 0.f;
-            for (int p = pb; p < pe; ++p) {
-                const float4 P4 = ptc[p];
-                const float2 A = tg.pa[p];
-                const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
-                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float rinv = __builtin_amdgcn_rsqf(d2);
-                const float d = d2 * rinv;
-
-                // hydrophobic contact (energy_prt_v1.cpp:44-50)
-                if (A.x != 0.0f) {
-                    const float u3 = fmaf(-d2, 0.5f / 81.0f / 81.0f / 81.0f / 81.0f, 2.5f / 81.0f / 81.0f / 81.0f);
-                    const float u2 = fmaf(-d2, u3, 4.5f / 81.0f / 81.0f);
-                    const float u1 = fmaf(-d2, u2, 3.5f / 81.0f);
-                    const float poly = fmaf(-d2, u1, 1.0f);
-                    sq = fmaf(A.x, d2 <= 81.0f ? poly : 0.0f, sq);
+                const int pe = min(pb + kBlock, e);
+                Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                switch (sg.w) { // segment flags: bit 0 hydrophobic points, bit 1 PSP points
+                case 0: pair_block<false, false>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
+                case 1: pair_block<true, false>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
+                case 2: pair_block<false, true>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
+                default: pair_block<true, true>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
                 }
-                // soft-core Lennard-Jones (:62-71)
-                const float r2 = rinv * rinv;
-                const float r3 = r2 * rinv;
-                const float r6 = r3 * r3;
-                const float pp1 = p1a * (r6 * r3);
-                const float pp2 = p2a * r6;
-                const float pp4 = fmaf(pp1 * sc.lj0, fmaf(sc.lj1, d2, 1.0f), 1.0f);
-                sv = fmaf(pp1 - pp2, __builtin_amdgcn_rcpf(pp4), sv);
-                // electrostatics (:79-85); charge of the atom applied after the loop
-                const float s1e = sc.el1 * d;
-                const float gpoly = fmaf(s1e * s1e, fmaf(sc.b1, s1e, sc.a1), sc.el0);
-                const float g1 = s1e < 1.0f ? gpoly : rinv * sc.inv_el1;
-                se = fmaf(P4.w, g1, se);
-                // contact potential (:95-98)
-                const float dm = d - pmf0;
-                sp = fmaf(pmf1, __builtin_amdgcn_rcpf(1.0f + fast_exp(fmaf(-0.5f, d, 6.0f) * dm)), sp);
-                // pocket-specific term (:107-109)
-                const int pslot = __float_as_int(A.y);
-                if (pslot >= 0) {
-                    if (dm <= 0.0f) ss += tg.pspc[pslot * GD_MAXTP2 + me.tl];
-                }
-                // hydrogen bond (:138-143); hdb1 == 0 where the pair has none
-                const float h3 = (d - hdb0) * hdb1;
-                sh = fmaf(hdb1, fast_exp(-0.5f * h3 * h3), sh);
-            }
-            acc_add<KAHAN>(evdw, cvdw, sv);
-            acc_add<KAHAN>(eele, cele, se);
-            acc_add<KAHAN>(epmf, cpmf, sp);
-            acc_add<KAHAN>(epsp, cpsp, ss);
-            acc_add<KAHAN>(ehdb, chdb, sh);
-            acc_add<KAHAN>(ehpc, chpc, sq);
+                acc_add<KAHAN>(evdw, cvdw, bs.v);
+                acc_add<KAHAN>(eele, cele, bs.e);
+                acc_add<KAHAN>(epmf, cpmf, bs.p);
+                acc_add<KAHAN>(epsp, cpsp, bs.s);
+                acc_add<KAHAN>(ehdb, chdb, bs.h);
+                acc_add<KAHAN>(ehpc, chpc, bs.q);
             }
         }
 
