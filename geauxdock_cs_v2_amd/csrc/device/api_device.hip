@@ -71,6 +71,12 @@ int ensure_records(gd_ctx *ctx, JobRes &j, int cap)
     return GD_OK;
 }
 
+// Device coordinates live in a frame centred on the target's pocket centre:
+// differences of two ~30 A absolute coordinates would lose ~2e-6 A to float
+// rounding, which the narrow KDE Gaussians and the normalisation slopes
+// amplify.  The shift is done in double and rounded once.
+float local(float x, float origin) { return (float)((double)x - (double)origin); }
+
 int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
               int waves, int max_rep, bool compensated, bool records)
 {
@@ -167,6 +173,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
 
     for (int i = 0; i < np; ++i)
         if (cx->prt_t[i] < 0 || cx->prt_t[i] >= GD_MAXTP1) return fail(GD_ERR_ARG, "protein point type out of range");
+    const float origin[3] = {cx->prt_pocket_center[0], cx->prt_pocket_center[1], cx->prt_pocket_center[2]};
 
     // per point: hydrophobic-contact weight (cdc ? hpp : 0) and the PSP row
     // of its residue (class-2 points of residues that carry PSP entries)
@@ -214,7 +221,8 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     for (int c = 0; c < nconf; ++c)
         for (int i = 0; i < np; ++i) {
             const size_t s = (size_t)c * np + ord[i];
-            pt[(size_t)c * np + i] = make_float4(cx->prt_x[s], cx->prt_y[s], cx->prt_z[s], cx->prt_ele[ord[i]]);
+            pt[(size_t)c * np + i] = make_float4(local(cx->prt_x[s], origin[0]), local(cx->prt_y[s], origin[1]),
+                                                 local(cx->prt_z[s], origin[2]), cx->prt_ele[ord[i]]);
         }
     std::vector<int4> seg;
     for (int i = 0; i < np;) {
@@ -255,7 +263,10 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.npad = np;
     t.dt.nseg = (int)seg.size();
     t.dt.nconf = nconf;
-    for (int d = 0; d < 3; ++d) t.dt.pocket[d] = cx->prt_pocket_center[d];
+    for (int d = 0; d < 3; ++d) {
+        t.origin[d] = origin[d];
+        t.dt.pocket[d] = 0.0f; // the pocket centre is the frame origin
+    }
     DevScalars &s = t.sc;
     s.lj0 = ep.lj0;
     s.lj1 = ep.lj1;
@@ -345,7 +356,9 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
         krange[i] = make_int2(b, e);
     }
     std::vector<float4> kde(z.kde_npoint);
-    for (int k = 0; k < z.kde_npoint; ++k) kde[k] = make_float4(cx->kde_x[k], cx->kde_y[k], cx->kde_z[k], 0.0f);
+    const float *org = tit->second.origin;
+    for (int k = 0; k < z.kde_npoint; ++k)
+        kde[k] = make_float4(local(cx->kde_x[k], org[0]), local(cx->kde_y[k], org[1]), local(cx->kde_z[k], org[2]), 0.0f);
     const int nent = cx->mcs_rowptr[z.mcs_nrow];
     std::vector<int> mrow(cx->mcs_rowptr, cx->mcs_rowptr + z.mcs_nrow + 1);
     std::vector<float4> ment(std::max(nent, 1));
@@ -354,7 +367,7 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
         if (ai < 0 || ai >= na) return fail(GD_ERR_ARG, "MCS atom index out of range");
         float w;
         std::memcpy(&w, &ai, 4);
-        ment[k] = make_float4(cx->mcs_x[k], cx->mcs_y[k], cx->mcs_z[k], w);
+        ment[k] = make_float4(local(cx->mcs_x[k], org[0]), local(cx->mcs_y[k], org[1]), local(cx->mcs_z[k], org[2]), w);
     }
     for (int m = 0; m < z.mcs_nrow; ++m)
         if (mrow[m + 1] <= mrow[m]) return fail(GD_ERR_ARG, "empty MCS row");
@@ -399,7 +412,7 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     h.n_lig = z.n_lig;
     h.n_prt = z.n_prt;
     h.n_tmp = z.n_tmp;
-    for (int d = 0; d < 3; ++d) h.center[d] = cx->lig_center[d];
+    for (int d = 0; d < 3; ++d) h.center[d] = local(cx->lig_center[d], org[d]);
     for (int k = 0; k < 6; ++k) h.scale[k] = cx->mcpara.move_scale[k];
     h.key0 = (uint32_t)seed;
     h.key1 = (uint32_t)(seed >> 32);

@@ -43,6 +43,7 @@ struct TargetRes {
     DevTarget dt{};
     DevScalars sc{};
     int n_prt = 0, npoint = 0;
+    float origin[3] = {0.f, 0.f, 0.f}; // frame origin (pocket centre) of all device coordinates
     std::vector<int> seg_host; // (type, begin, end) triples
     size_t bytes = 0;
 };
