@@ -66,7 +66,19 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (scales rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_calibration} MiB x 5")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
               "(replica ladders shard by job across GPUs with no collective in the MC loop)", file=sys.stderr)
@@ -101,20 +113,22 @@ def main():
     dev.free_job(j)
 
     # plain form: reference structure on a window of the same workload
-    jp = dev.upload_job(t, cx, seed=2024)
-    t0 = time.perf_counter()
-    for s1 in range(0, args.plain_steps, 100):
-        s2 = min(100, args.plain_steps - s1)
-        dev.montecarlo(jp, s1, s2)
-        dev.records(jp, n_rep, 100)
-        if exch and (s1 + s2) % exch == 0:
-            dev.exchange(jp, (s1 + s2) // exch - 1)
-    plain_wall = time.perf_counter() - t0
-    plain_value = n_rep * args.plain_steps / plain_wall
-    dev.free_job(jp)
+    plain_value = None
+    if args.plain_steps > 0:
+        jp = dev.upload_job(t, cx, seed=2024)
+        t0 = time.perf_counter()
+        for s1 in range(0, args.plain_steps, 100):
+            s2 = min(100, args.plain_steps - s1)
+            dev.montecarlo(jp, s1, s2)
+            dev.records(jp, n_rep, 100)
+            if exch and (s1 + s2) % exch == 0:
+                dev.exchange(jp, (s1 + s2) // exch - 1)
+        plain_wall = time.perf_counter() - t0
+        plain_value = n_rep * args.plain_steps / plain_wall
+        dev.free_job(jp)
+        log(f"plain: {args.plain_steps} steps in {plain_wall:.3f} s, {plain_value:.4g} poses/s")
     dev.free_target(t)
     dev.close()
-    log(f"plain: {args.plain_steps} steps in {plain_wall:.3f} s, {plain_value:.4g} poses/s")
 
     cpu = None if args.no_cpu else cpu_baseline(cx, args.cpu_seconds)
     if cpu:
@@ -130,7 +144,7 @@ def main():
                    "replicas": n_rep, "form": "batched (packed replicas, autotuned shape, hipGraph chunks)"},
         "plain": {"value": plain_value, "steps": args.plain_steps,
                   "form": "one workgroup per replica, 100-step launches, record read-back per chunk"},
-        "speedup_vs_plain": value / plain_value,
+        "speedup_vs_plain": value / plain_value if plain_value else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": perf.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / perf.FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "mc_kernel", "launch_ms": launch_ms, "flop_per_pose": f_pose,

@@ -175,34 +175,16 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         if (cx->prt_t[i] < 0 || cx->prt_t[i] >= GD_MAXTP1) return fail(GD_ERR_ARG, "protein point type out of range");
     const float origin[3] = {cx->prt_pocket_center[0], cx->prt_pocket_center[1], cx->prt_pocket_center[2]};
 
-    // per point: hydrophobic-contact weight (cdc ? hpp : 0) and the PSP row
-    // of its residue (class-2 points of residues that carry PSP entries)
-    std::map<int, int> slot_of;
-    std::vector<float> pspc;
+    // per point: hydrophobic-contact weight (cdc ? hpp : 0); points are
+    // re-sorted by (type, hydrophobic flag), keeping the class order
+    // otherwise, so a segment shares one set of type-pair parameters and one
+    // term set
     std::vector<float> hpcw(np);
-    std::vector<int> slot(np, -1), flags(np);
+    std::vector<int> flags(np);
     for (int s = 0; s < np; ++s) {
         hpcw[s] = cx->prt_cdc[s] == 1 ? cx->prt_hpp[s] : 0.0f;
-        if (cx->prt_c[s] == 2) {
-            const int res = cx->prt_seq3r[s];
-            if (res < 0 || res >= cx->psp_width) return fail(GD_ERR_ARG, "residue serial outside the PSP table");
-            auto it = slot_of.find(res);
-            if (it != slot_of.end()) {
-                slot[s] = it->second;
-            } else {
-                bool any = false;
-                for (int t = 0; t < GD_MAXTP2; ++t) any |= cx->psp[(size_t)t * cx->psp_width + res] != 0.0f;
-                if (any) {
-                    slot[s] = (int)slot_of.size();
-                    slot_of[res] = slot[s];
-                    for (int t = 0; t < GD_MAXTP2; ++t) pspc.push_back(cx->psp[(size_t)t * cx->psp_width + res]);
-                }
-            }
-        }
-        flags[s] = (hpcw[s] != 0.0f ? 1 : 0) | (slot[s] >= 0 ? 2 : 0);
+        flags[s] = hpcw[s] != 0.0f ? 1 : 0;
     }
-    // points re-sorted by (type, flags), keeping the class order otherwise:
-    // a segment shares one set of type-pair parameters and one term set
     std::vector<int> ord(np);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
@@ -214,7 +196,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         const int s = ord[i];
         float2 v;
         v.x = hpcw[s];
-        std::memcpy(&v.y, &slot[s], 4);
+        std::memcpy(&v.y, &cx->prt_t[s], 4);
         pa[i] = v;
     }
     std::vector<float4> pt((size_t)nconf * np);
@@ -245,28 +227,22 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             q[PP_HDB1] = ep.hdb0[tl][tp] > 0.1f ? ep.hdb1[tl][tp] : 0.0f;
             q[PP_HDB1S] = q[PP_HDB1] * hdb_scale;
         }
-    if (pspc.empty()) pspc.assign(GD_MAXTP2, 0.0f);
 
     const int id = ctx->next_id++;
     TargetRes &t = ctx->targets[id];
     GD_HIP(upload(t.pt, pt.data(), sizeof(float4) * pt.size(), ctx->stream));
     GD_HIP(upload(t.pa, pa.data(), sizeof(float2) * pa.size(), ctx->stream));
     GD_HIP(upload(t.pp, pp.data(), sizeof(float) * pp.size(), ctx->stream));
-    GD_HIP(upload(t.pspc, pspc.data(), sizeof(float) * pspc.size(), ctx->stream));
     GD_HIP(upload(t.seg, seg.data(), sizeof(int4) * seg.size(), ctx->stream));
     t.dt.pt = t.pt.as<float4>();
     t.dt.pa = t.pa.as<float2>();
     t.dt.pp = t.pp.as<float>();
-    t.dt.pspc = t.pspc.as<float>();
     t.dt.seg = t.seg.as<int4>();
     t.dt.npoint = np;
     t.dt.npad = np;
     t.dt.nseg = (int)seg.size();
     t.dt.nconf = nconf;
-    for (int d = 0; d < 3; ++d) {
-        t.origin[d] = origin[d];
-        t.dt.pocket[d] = 0.0f; // the pocket centre is the frame origin
-    }
+    for (int d = 0; d < 3; ++d) t.origin[d] = origin[d];
     DevScalars &s = t.sc;
     s.lj0 = ep.lj0;
     s.lj1 = ep.lj1;
@@ -294,7 +270,11 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         t.seg_host.push_back(g.y);
         t.seg_host.push_back(g.z);
     }
-    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.pspc.bytes + t.seg.bytes;
+    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.seg.bytes;
+    t.dev_index.assign(np, 0);
+    for (int i = 0; i < np; ++i) t.dev_index[ord[i]] = i;
+    t.prt_t.assign(cx->prt_t, cx->prt_t + np);
+    t.prt_c.assign(cx->prt_c, cx->prt_c + np);
     GD_HIP(hipStreamSynchronize(ctx->stream));
     *target_id = id;
     return GD_OK;
@@ -316,7 +296,8 @@ size_t gd_job_footprint(int lig_natom, int n_lig, int kde_npoint, int mcs_entrie
     return 2 * sizeof(float4) * a * n_lig + (sizeof(int) + sizeof(int2)) * a + sizeof(float4) * kde_npoint +
            sizeof(int) * (mcs_nrow + 1) + sizeof(float4) * mcs_entries + sizeof(float) * mcs_nrow +
            sizeof(float) * n_tmp + sizeof(gd_ReplicaMC) * (size_t)n_rep * (1 + record_cap) +
-           sizeof(int) * (size_t)n_rep + sizeof(DevJob) + 9 * 256; // allocation granularity slack
+           sizeof(int) * (size_t)n_rep + sizeof(DevJob) + 11 * 256 // allocation granularity slack
+           + 64 * 1024; // PSP point rows (100 bytes per point, up to ~650 points)
 }
 
 int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t seed, int *job_id)
@@ -375,6 +356,31 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     std::vector<float> mbeta(z.n_tmp);
     for (int t = 0; t < z.n_tmp; ++t) mbeta[t] = cx->temp[t].minus_beta;
 
+    // the job's PSP points (the PSP table comes with the ligand's .ff file):
+    // class-2 points whose residue has a nonzero entry, in device order
+    const TargetRes &tr = tit->second;
+    const int np = z.prt_npoint;
+    for (int s = 0; s < np; ++s)
+        if (cx->prt_t[s] != tr.prt_t[s] || cx->prt_c[s] != tr.prt_c[s])
+            return fail(GD_ERR_ARG, "job protein topology differs from the resident target");
+    std::vector<int> psp_pt;
+    std::vector<float> psp_row;
+    for (int s = 0; s < np; ++s) {
+        if (cx->prt_c[s] != 2) continue;
+        const int res = cx->prt_seq3r[s];
+        if (res < 0 || res >= cx->psp_width) return fail(GD_ERR_ARG, "residue serial outside the PSP table");
+        bool any = false;
+        for (int t = 0; t < GD_MAXTP2; ++t) any |= cx->psp[(size_t)t * cx->psp_width + res] != 0.0f;
+        if (!any) continue;
+        psp_pt.push_back(tr.dev_index[s]);
+        for (int t = 0; t < GD_MAXTP2; ++t) psp_row.push_back(cx->psp[(size_t)t * cx->psp_width + res]);
+    }
+    const int npsp = (int)psp_pt.size();
+    if (psp_pt.empty()) {
+        psp_pt.push_back(0);
+        psp_row.assign(GD_MAXTP2, 0.0f);
+    }
+
     const int id = ctx->next_id++;
     JobRes &j = ctx->jobs[id];
     j.target = target_id;
@@ -388,6 +394,8 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     GD_HIP(upload(j.ment, ment.data(), sizeof(float4) * ment.size(), s));
     GD_HIP(upload(j.mtcc, mtcc.data(), sizeof(float) * mtcc.size(), s));
     GD_HIP(upload(j.mbeta, mbeta.data(), sizeof(float) * mbeta.size(), s));
+    GD_HIP(upload(j.psp_pt, psp_pt.data(), sizeof(int) * psp_pt.size(), s));
+    GD_HIP(upload(j.psp_row, psp_row.data(), sizeof(float) * psp_row.size(), s));
     GD_HIP(upload(j.rep, cx->replica, sizeof(gd_ReplicaMC) * z.n_rep, s));
     GD_HIP(j.next_entry.alloc(sizeof(int) * z.n_rep));
     GD_HIP(hipMemsetAsync(j.next_entry.p, 0, sizeof(int) * z.n_rep, s));
@@ -402,6 +410,9 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     h.ment = j.ment.as<float4>();
     h.mtcc = j.mtcc.as<float>();
     h.mbeta = j.mbeta.as<float>();
+    h.psp_pt = j.psp_pt.as<int>();
+    h.psp_row = j.psp_row.as<float>();
+    h.npsp = npsp;
     h.rep = j.rep.as<gd_ReplicaMC>();
     h.rec = nullptr;
     h.next_entry = j.next_entry.as<int>();
@@ -412,7 +423,10 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     h.n_lig = z.n_lig;
     h.n_prt = z.n_prt;
     h.n_tmp = z.n_tmp;
-    for (int d = 0; d < 3; ++d) h.center[d] = local(cx->lig_center[d], org[d]);
+    for (int d = 0; d < 3; ++d) {
+        h.center[d] = local(cx->lig_center[d], org[d]);
+        h.pocket[d] = local(cx->prt_pocket_center[d], org[d]);
+    }
     for (int k = 0; k < 6; ++k) h.scale[k] = cx->mcpara.move_scale[k];
     h.key0 = (uint32_t)seed;
     h.key1 = (uint32_t)(seed >> 32);
@@ -434,7 +448,8 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
         }
     }
     j.bytes = j.lig3.bytes + j.lig2.bytes + j.lt.bytes + j.krange.bytes + j.kde.bytes + j.mrow.bytes + j.ment.bytes +
-              j.mtcc.bytes + j.mbeta.bytes + j.rep.bytes + j.next_entry.bytes + j.self.bytes;
+              j.mtcc.bytes + j.mbeta.bytes + j.psp_pt.bytes + j.psp_row.bytes + j.rep.bytes + j.next_entry.bytes +
+              j.self.bytes;
     GD_HIP(hipStreamSynchronize(s));
     *job_id = id;
     return GD_OK;

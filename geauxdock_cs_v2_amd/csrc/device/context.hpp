@@ -39,18 +39,20 @@ struct DevBuf {
 };
 
 struct TargetRes {
-    DevBuf pt, pa, pp, pspc, seg;
+    DevBuf pt, pa, pp, seg;
     DevTarget dt{};
     DevScalars sc{};
     int n_prt = 0, npoint = 0;
     float origin[3] = {0.f, 0.f, 0.f}; // frame origin (pocket centre) of all device coordinates
     std::vector<int> seg_host; // (type, begin, end) triples
+    std::vector<int> dev_index;     // device position of each point (staged class order)
+    std::vector<int> prt_t, prt_c;  // staged point types and classes, to check jobs against
     size_t bytes = 0;
 };
 
 struct JobRes {
     int target = -1;
-    DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, rep, rec, next_entry, self;
+    DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, psp_pt, psp_row, rep, rec, next_entry, self;
     DevJob hj{};       // host copy of the descriptor
     int n_rep = 0, n_lig = 0, n_prt = 0, n_tmp = 0, natom = 0, nrow = 0;
     std::vector<int> rep_prt; // protein conformation of every replica (never changes)

@@ -2,21 +2,23 @@
 // uploads and launches them.
 //
 // HBM layout (one resident target, any number of jobs):
-//   target  pt   float4[nconf][npad]  {x, y, z, ele} of every effective point,
-//                                     re-sorted by point type (stable within a
-//                                     type) so a run of points shares one set
-//                                     of type-pair parameters
-//           pa   float2[npad]         {hpc weight = cdc ? hpp : 0, PSP slot}
-//           seg  int4[nseg]           {point type, begin, end, flags}: flags bit 0 =
-//                                     hydrophobic points, bit 1 = PSP points
+//   target  pt   float4[nconf][npad]  {x, y, z, ele} of every effective point
+//                                     in a frame centred on the pocket centre,
+//                                     re-sorted by (point type, hydrophobic
+//                                     flag) so a run of points shares one set
+//                                     of type-pair parameters and terms
+//           pa   float2[npad]         {hpc weight = cdc ? hpp : 0, point type}
+//           seg  int4[nseg]           {point type, begin, end, hydrophobic flag}
 //           pp   float[31][24][8]     type-pair parameters, see PairParam
-//           pspc float[nslot][24]     PSP rows of residues that have entries
 //   job     lig3 float4[nlig][natom]  {x3, y3, z3, c3} type-sorted atoms
 //           lig2 float4[nlig][natom]  {x2, y2, z2, 0} file-order atoms (MCS)
 //           lt   int[natom]           t3
 //           krange int2[natom]        same-type range in the sorted KDE table
 //           kde  float4[nkde]         {x, y, z, 0}
 //           mrow int[nrow+1], ment float4[nent] {x, y, z, file atom index}, mtcc
+//           psp_pt int[npsp], psp_row float[npsp][24]: the job's class-2 points
+//                                     whose residue has PSP entries (the PSP
+//                                     table comes with the ligand's .ff file)
 //           rep  gd_ReplicaMC[n_rep], rec gd_ReplicaMC[n_rep][rec_cap], next_entry
 #pragma once
 
@@ -46,14 +48,11 @@ struct DevScalars {
 };
 
 struct DevTarget {
-    const float4 *pt;
-    const float2 *pa;
-    const float *pp;
-    const float *pspc;
-    const int4 *seg;
+    const float4 *pt;  // [nconf][npad] {x, y, z, ele}, pocket-centred frame
+    const float2 *pa;  // [npad] {hydrophobic weight, point type (int bits)}
+    const float *pp;   // [31][24][PP_N] type-pair parameters
+    const int4 *seg;   // [nseg] {point type, begin, end, hydrophobic flag}
     int npoint, npad, nseg, nconf;
-    float pocket[3];
-    int pad;
 };
 
 struct DevJob {
@@ -66,12 +65,15 @@ struct DevJob {
     const float4 *ment;
     const float *mtcc;
     const float *mbeta;
+    const int *psp_pt;     // [npsp] device point index of each PSP point
+    const float *psp_row;  // [npsp][24] PSP value per ligand type
     gd_ReplicaMC *rep;
     gd_ReplicaMC *rec;
     int *next_entry;
     int natom, nrow, rec_cap, n_rep;
-    int n_lig, n_prt, n_tmp, pad;
-    float center[3];
+    int n_lig, n_prt, n_tmp, npsp;
+    float center[3]; // ligand centre, target frame
+    float pocket[3]; // the job's pocket centre, target frame
     float scale[6];
     uint32_t key0, key1;
 };

@@ -87,20 +87,20 @@ struct PairParams {
 };
 
 struct Sums {
-    float v, e, p, s, h, q; // vdw, ele (without the atom charge), pmf, psp, hdb, hydrophobic
+    float v, e, p, h, q; // vdw, ele (without the atom charge), pmf, hdb, hydrophobic
 };
 
 constexpr float kLog2e = 1.4426950408889634f;
 
-// Six pair terms of one atom against points [pb, pe) of one type segment
-// (src/backend_cpu_mic/energy_prt_v1.cpp:30-146).  HPC: every point of the
-// segment is a hydrophobic-contact point (cdc with nonzero hpp); PSP: every
-// point carries a PSP row.  Points without them live in other segments, so
-// the loop needs no per-point branch for either term.
-template <bool HPC, bool PSP>
-__device__ __forceinline__ void pair_block(const float4 *__restrict__ ptc, const float2 *__restrict__ pa,
-                                           const float *__restrict__ pspc, int pb, int pe, const Slot &me,
-                                           const PairParams &q, const DevScalars &sc, Sums &s)
+// Pair terms of one atom against points [pb, pe) of one type segment
+// (src/backend_cpu_mic/energy_prt_v1.cpp:30-146) except the pocket-specific
+// term, which the job's short PSP point list adds separately.  HPC: every
+// point of the segment is a hydrophobic-contact point (cdc with nonzero hpp);
+// the other points live in other segments, so the loop has no per-point
+// branch.
+template <bool HPC>
+__device__ __forceinline__ void pair_block(const float4 *__restrict__ ptc, const float2 *__restrict__ pa, int pb,
+                                           int pe, const Slot &me, const PairParams &q, const DevScalars &sc, Sums &s)
 {
 #pragma unroll 2
     for (int p = pb; p < pe; ++p) {
@@ -134,10 +134,6 @@ __device__ __forceinline__ void pair_block(const float4 *__restrict__ ptc, const
         const float dm = d - q.pmf0;
         const float xe = fmaf(-0.5f * kLog2e, d, 6.0f * kLog2e) * dm;
         s.p = fmaf(q.pmf1, __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(xe)), s.p);
-        // pocket-specific term where d <= pmf0 (:107-109)
-        if (PSP) {
-            if (dm <= 0.0f) s.s += pspc[__float_as_int(pa[p].y) * GD_MAXTP2 + me.tl];
-        }
         // hydrogen bond hdb1 exp(-((d - hdb0) hdb1)^2 / 2), hdb1 == 0 where
         // the pair has none (:138-143); hdb1s = hdb1 sqrt(log2(e) / 2)
         const float h3 = (d - q.hdb0) * q.hdb1s;
@@ -280,20 +276,35 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
             // 1e4 over a pose, which long running FP32 sums cannot hold
             for (int pb = b; pb < e; pb += kBlock) {
                 const int pe = min(pb + kBlock, e);
-                Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                switch (sg.w) { // segment flags: bit 0 hydrophobic points, bit 1 PSP points
-                case 0: pair_block<false, false>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
-                case 1: pair_block<true, false>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
-                case 2: pair_block<false, true>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
-                default: pair_block<true, true>(ptc, tg.pa, tg.pspc, pb, pe, me, q, sc, bs); break;
-                }
+                Sums bs{0.f, 0.f, 0.f, 0.f, 0.f};
+                if (sg.w) // segment flag: hydrophobic-contact points
+                    pair_block<true>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                else
+                    pair_block<false>(ptc, tg.pa, pb, pe, me, q, sc, bs);
                 acc_add<KAHAN>(evdw, cvdw, bs.v);
                 acc_add<KAHAN>(eele, cele, bs.e);
                 acc_add<KAHAN>(epmf, cpmf, bs.p);
-                acc_add<KAHAN>(epsp, cpsp, bs.s);
                 acc_add<KAHAN>(ehdb, chdb, bs.h);
                 acc_add<KAHAN>(ehpc, chpc, bs.q);
             }
+        }
+        // pocket-specific term: the job's class-2 points whose residue has a
+        // PSP row add psp[t_l][residue] where d <= pmf0 (energy_prt_v1.cpp:107-109);
+        // d is computed exactly as in the pair loop
+        {
+            float ss = 0.f;
+            for (int k = 0; k < J.npsp; ++k) {
+                const int p = J.psp_pt[k];
+                if (p < lo || p >= hi) continue; // the slice that owns the point adds it
+                const float4 P4 = ptc[p];
+                const int tp = __float_as_int(tg.pa[p].y);
+                const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float d = d2 * __builtin_amdgcn_rsqf(d2);
+                const float pmf0 = tg.pp[((size_t)tp * GD_MAXTP2 + me.tl) * PP_N + PP_PMF0];
+                if (d - pmf0 <= 0.0f) ss += J.psp_row[k * GD_MAXTP2 + me.tl];
+            }
+            acc_add<KAHAN>(epsp, cpsp, ss);
         }
 
         // ---- cross-slice sum ----
@@ -385,10 +396,10 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
             // |centre + m - pocket|: the reference adds m to the ~30 A centre
             // first and loses the small offset to rounding; subtracting the
             // two centres first is exact when they coincide (they do: both
-            // are the native ligand centroid)
+            // are the job's native ligand centroid)
             float dst = 0.f;
             for (int k = 0; k < 3; ++k) {
-                const float dd = (J.center[k] - tg.pocket[k]) + pm[k];
+                const float dd = (J.center[k] - J.pocket[k]) + pm[k];
                 dst += dd * dd;
             }
             e[GD_E_DST] = sqrtf(dst);

@@ -66,12 +66,18 @@ struct JobCopy {
         s.mcs_z.assign(cx->mcs_z, cx->mcs_z + nent);
         s.temp.assign(cx->temp, cx->temp + z.n_tmp);
         s.replica.assign(cx->replica, cx->replica + z.n_rep);
-        s.view = *cx; // sizes, centre, enepara, mcpara
-        s.view.prt_x = s.view.prt_y = s.view.prt_z = nullptr;
-        s.view.prt_hpp = s.view.prt_ele = nullptr;
-        s.view.prt_t = s.view.prt_c = s.view.prt_cdc = s.view.prt_seq3r = nullptr;
-        s.view.psp = nullptr;
+        // protein topology of the first conformation (checked against the
+        // target) and the job's own PSP table
+        const int np = z.prt_npoint;
+        s.prt_t.assign(cx->prt_t, cx->prt_t + np);
+        s.prt_c.assign(cx->prt_c, cx->prt_c + np);
+        s.prt_seq3r.assign(cx->prt_seq3r, cx->prt_seq3r + np);
+        s.psp.assign(cx->psp, cx->psp + (size_t)GD_MAXTP2 * cx->psp_width);
+        s.view = *cx; // sizes, centres, psp_width, enepara, mcpara
         s.bind();
+        s.view.prt_x = s.view.prt_y = s.view.prt_z = nullptr; // coordinates live in the target
+        s.view.prt_hpp = s.view.prt_ele = nullptr;
+        s.view.prt_cdc = nullptr;
     }
 };
 
