@@ -265,3 +265,38 @@ def test_compensated_accumulation_within_tolerance(lib, ctx):
     err = rel_err(eg, e64).max(0)
     say("compensated max rel err vs FP64:", np.array2string(err, precision=2))
     assert err.max() <= REL_TOL
+
+
+def test_two_ligands_one_resident_target(lib, ctx, tmp_path):
+    """Two ligand jobs with their own .ff files (own KDE, MCS and PSP tables,
+    own pocket centre) docked against one resident target, advanced together
+    in one batched launch: energies of both against the FP64 oracle."""
+    from geauxdock_cs_v2_amd import synth
+    a = synth.generate(str(tmp_path), "jA", seed=3, n_atoms=21, n_lig_conf=3, lig_sigma=0.4, n_points=900,
+                       n_prt_conf=2, prt_sigma=0.4, prt_radius=18.0, n_kde=80, mcs_rows=5, mcs_cols=8)
+    b = synth.generate(str(tmp_path), "jB", seed=4, n_atoms=40, n_lig_conf=4, lig_sigma=0.4, n_points=900,
+                       n_prt_conf=2, prt_sigma=0.4, prt_radius=18.0, n_kde=120, mcs_rows=7, mcs_cols=9,
+                       protein=a["protein"])
+    cxs = [L.load_job(v["pdb"], v["sdf"], v["ff"], k, n_tmp=3, temp_high=2.0, temp_low=0.3).stage()
+           for k, v in (("jA", a), ("jB", b))]
+    t = C.c_int()
+    L.check(lib.gd_upload_target(ctx, C.byref(cxs[0].view), C.byref(t)))
+    jobs = []
+    for i, cx in enumerate(cxs):
+        j = C.c_int()
+        L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), 100 + i, C.byref(j)))
+        jobs.append(j.value)
+    say("case: two ligands, one target, batched 40 steps")
+    _batch(lib, ctx, jobs, 40, waves=8)
+    for cx, j in zip(cxs, jobs):
+        n = cx.size["n_rep"]
+        g = read_reps(lib, ctx, j, n)
+        idx = np.arange(n, dtype=np.int32)
+        mm = np.array([list(r.movematrix)[:6] for r in g], np.float32)
+        e64 = O.energy(cx.view, idx, mm, "f64")
+        eg = np.array([list(r.energy)[:10] for r in g])
+        err = rel_err(eg, e64).max(0)
+        say("two-job max rel err:", np.array2string(err, precision=2))
+        assert err.max() <= REL_TOL
+        L.check(lib.gd_free_job(ctx, j))
+    L.check(lib.gd_free_target(ctx, t))
