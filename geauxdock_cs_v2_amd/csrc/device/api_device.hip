@@ -195,7 +195,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     for (int i = 0; i < np; ++i) {
         const int s = ord[i];
         float2 v;
-        v.x = hpcw[s];
+        v.x = 0.5f * hpcw[s]; // the factored contact polynomial leaves a factor 1/2 to the weight
         std::memcpy(&v.y, &cx->prt_t[s], 4);
         pa[i] = v;
     }
@@ -214,18 +214,28 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         seg.push_back(make_int4(t, i, j, f));
         i = j;
     }
-    const float hdb_scale = std::sqrt(0.5f * 1.4426950408889634f); // folds exp(-x^2/2) into exp2
+    // exponents as quadratics in d, coefficients rounded once from double
+    const double log2e = 1.4426950408889634;
+    const double ljc = ep.lj0 != 0.0f ? (double)ep.lj0 : 1.0; // LJ sums are kept times ljc
     std::vector<float> pp((size_t)GD_MAXTP1 * GD_MAXTP2 * PP_N, 0.0f);
     for (int tp = 0; tp < GD_MAXTP1; ++tp)
         for (int tl = 0; tl < GD_MAXTP2; ++tl) {
             float *q = &pp[((size_t)tp * GD_MAXTP2 + tl) * PP_N];
-            q[PP_P1A] = ep.p1a[tl][tp];
-            q[PP_P2A] = ep.p2a[tl][tp];
+            const double pmf0 = ep.pmf0[tl][tp];
+            q[PP_P1A] = (float)(ljc * ep.p1a[tl][tp]);
+            q[PP_P2A] = (float)(ljc * ep.p2a[tl][tp]);
             q[PP_PMF0] = ep.pmf0[tl][tp];
             q[PP_PMF1] = ep.pmf1[tl][tp];
+            q[PP_PMFB] = (float)(log2e * (6.0 + 0.5 * pmf0));
+            q[PP_PMFC] = (float)(-6.0 * log2e * pmf0);
+            const bool hb = ep.hdb0[tl][tp] > 0.1f; // energy_prt_v1.cpp:138
+            const double h0 = ep.hdb0[tl][tp], h1 = hb ? (double)ep.hdb1[tl][tp] : 0.0;
+            const double k = 0.5 * log2e * h1 * h1;
+            q[PP_HK2] = (float)(-k);
+            q[PP_HK1] = (float)(2.0 * k * h0);
+            q[PP_HK0] = (float)(-k * h0 * h0);
+            q[PP_HDB1] = hb ? ep.hdb1[tl][tp] : 0.0f;
             q[PP_HDB0] = ep.hdb0[tl][tp];
-            q[PP_HDB1] = ep.hdb0[tl][tp] > 0.1f ? ep.hdb1[tl][tp] : 0.0f;
-            q[PP_HDB1S] = q[PP_HDB1] * hdb_scale;
         }
 
     const int id = ctx->next_id++;
@@ -244,13 +254,18 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.nconf = nconf;
     for (int d = 0; d < 3; ++d) t.origin[d] = origin[d];
     DevScalars &s = t.sc;
-    s.lj0 = ep.lj0;
-    s.lj1 = ep.lj1;
-    s.el0 = ep.el0;
-    s.el1 = ep.el1;
-    s.a1 = ep.a1;
-    s.b1 = ep.b1;
+    s.lj_a = (float)(ep.lj0 / ljc);
+    s.lj_b = (float)(ep.lj0 / ljc * ep.lj1);
+    s.inv_lj0 = (float)(1.0 / ljc);
     s.inv_el1 = 1.0f / ep.el1;
+    {
+        const double e1 = ep.el1;
+        s.el_e0 = (float)(e1 * ep.el0);
+        s.el_a = (float)((double)ep.a1 * e1 * e1 * e1);
+        s.el_b = (float)((double)ep.b1 * e1 * e1 * e1 * e1);
+        s.el_dc2 = (float)(1.0 / (e1 * e1));
+        s.pmf_a = (float)(-0.5 * log2e);
+    }
     s.kde2 = ep.kde2;
     s.kde3inv = 1.0f / ep.kde3;
     for (int i = 0; i < GD_MAXTP2; ++i) {

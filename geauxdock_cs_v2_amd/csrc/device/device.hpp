@@ -7,9 +7,9 @@
 //                                     re-sorted by (point type, hydrophobic
 //                                     flag) so a run of points shares one set
 //                                     of type-pair parameters and terms
-//           pa   float2[npad]         {hpc weight = cdc ? hpp : 0, point type}
+//           pa   float2[npad]         {hpc weight = (cdc ? hpp : 0) / 2, point type}
 //           seg  int4[nseg]           {point type, begin, end, hydrophobic flag}
-//           pp   float[31][24][8]     type-pair parameters, see PairParam
+//           pp   float[31][24][PP_N]  type-pair parameters, see PairParam
 //   job     lig3 float4[nlig][natom]  {x3, y3, z3, c3} type-sorted atoms
 //           lig2 float4[nlig][natom]  {x2, y2, z2, 0} file-order atoms (MCS)
 //           lt   int[natom]           t3
@@ -29,27 +29,42 @@
 
 namespace gd {
 
-// per (point type, ligand type) parameters, 8 floats (32 bytes)
+// per (point type, ligand type) parameters, PP_N floats.  The contact and
+// H-bond exponents are stored as quadratics in d so a pair needs two FMAs
+// for each: log2(e) (6 - d/2)(d - pmf0) = pmf_a d^2 + PMFB d + PMFC and
+// -log2(e) ((d - hdb0) hdb1)^2 / 2 = HK2 d^2 + HK1 d + HK0.
 enum PairParam {
-    PP_P1A = 0,  // LJ repulsive prefactor p1a
-    PP_P2A = 1,  // LJ attractive prefactor p2a
+    PP_P1A = 0,  // LJ repulsive prefactor p1a, times c (DevScalars)
+    PP_P2A = 1,  // LJ attractive prefactor p2a, times c
     PP_PMF0 = 2, // contact distance
     PP_PMF1 = 3, // contact depth
-    PP_HDB0 = 4, // H-bond distance
-    PP_HDB1 = 5, // H-bond inverse width, 0 where the pair has no H-bond term
-    PP_HDB1S = 6, // PP_HDB1 * sqrt(log2(e) / 2)
-    PP_N = 8
+    PP_PMFB = 4, // log2(e) (6 + pmf0 / 2)
+    PP_PMFC = 5, // -6 log2(e) pmf0
+    PP_HK2 = 6,  // -log2(e) hdb1^2 / 2
+    PP_HK1 = 7,  // log2(e) hdb1^2 hdb0
+    PP_HK0 = 8,  // -log2(e) hdb1^2 hdb0^2 / 2
+    PP_HDB1 = 9, // H-bond inverse width, 0 where the pair has no H-bond term
+    PP_HDB0 = 10, // H-bond distance
+    PP_N = 12
 };
 
 struct DevScalars {
-    float lj0, lj1, el0, el1, a1, b1, inv_el1, kde2, kde3inv;
+    // LJ sums are kept times c = lj0 (1 where lj0 = 0); p1a and p2a in the
+    // pair table carry the factor: pp4 = 1 + X (lj_a + lj_b d2), X = c pp1
+    float lj_a, lj_b, inv_lj0;
+    float inv_el1, kde2, kde3inv;
+    // electrostatics inside d < 1 / el1, scaled by el1 (the sum is scaled
+    // back once per atom): el1 (el0 + a1 s^2 + b1 s^3), s = el1 d, is
+    // el_e0 + el_a d^2 + el_b d^3
+    float el_e0, el_a, el_b, el_dc2;
+    float pmf_a; // -log2(e) / 2
     float hpl0[GD_MAXTP2], hpl1[GD_MAXTP2], hpl2[GD_MAXTP2];
     float a_para[GD_MAXWEI], b_para[GD_MAXWEI], w[GD_MAXWEI];
 };
 
 struct DevTarget {
     const float4 *pt;  // [nconf][npad] {x, y, z, ele}, pocket-centred frame
-    const float2 *pa;  // [npad] {hydrophobic weight, point type (int bits)}
+    const float2 *pa;  // [npad] {hydrophobic weight / 2, point type (int bits)}
     const float *pp;   // [31][24][PP_N] type-pair parameters
     const int4 *seg;   // [nseg] {point type, begin, end, hydrophobic flag}
     int npoint, npad, nseg, nconf;

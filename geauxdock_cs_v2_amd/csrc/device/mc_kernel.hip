@@ -83,61 +83,112 @@ constexpr int kBlock = 64; // points per plain FP32 partial sum
 namespace {
 
 struct PairParams {
-    float p1a, p2a, pmf0, pmf1, hdb0, hdb1, hdb1s;
+    float p1a, p2a, pmf1, pmfb, pmfc, hk2, hk1, hk0, hdb1;
 };
 
 struct Sums {
-    float v, e, p, h, q; // vdw, ele (without the atom charge), pmf, hdb, hydrophobic
+    float v, e, p, h, q; // vdw (lj0-scaled), ele (el1-scaled, without the atom charge), pmf, hdb, hydrophobic
 };
 
-constexpr float kLog2e = 1.4426950408889634f;
+// The target tables are read-only while a kernel runs: reading them through
+// the constant address space lets the compiler fetch wave-uniform points with
+// scalar loads into SGPRs instead of 64 identical vector loads.
+typedef const __attribute__((address_space(4))) float cfloat;
+
+__device__ __forceinline__ float4 ld_point(const cfloat *pt, int p)
+{
+    return make_float4(pt[4 * p], pt[4 * p + 1], pt[4 * p + 2], pt[4 * p + 3]);
+}
+
+// Electrostatics (:79-85), times el1: el1 g1 is 1/d outside the core and
+// el1 (el0 + a1 s^2 + b1 s^3), s = el1 d, inside s < 1 (the core meets 1/s
+// with equal value and slope at s = 1).  Every pair adds ele / d; a pair
+// inside the core then adds ele (core - 1/d).
+__device__ __forceinline__ float ele_core_delta(float d2, float rinv, float ele, const DevScalars &sc)
+{
+    const float d = d2 * rinv;
+    const float core = fmaf(d2, fmaf(sc.el_b, d, sc.el_a), sc.el_e0);
+    return d2 < sc.el_dc2 ? ele * (core - rinv) : 0.0f;
+}
+
+// Terms of one pair once d2 and 1/d are known (electrostatics outside the
+// core only).
+template <bool HPC, bool HDB>
+__device__ __forceinline__ void pair_terms(float d2, float rinv, float ele, float hw, const PairParams &q,
+                                           const DevScalars &sc, Sums &s)
+{
+    const float d = d2 * rinv;
+    // hydrophobic contact inside 9 A (:44-50): with t = d2 / 81,
+    // 1 - 3.5t + 4.5t^2 - 2.5t^3 + 0.5t^4 = (1-t)^3 (1 - t/2) = u^3 (1 + u) / 2
+    // for u = 1 - t; u clamps to 0 beyond the cutoff, the weight holds the 1/2
+    if (HPC) {
+        const float u = __builtin_amdgcn_fmed3f(fmaf(-d2, 1.0f / 81.0f, 1.0f), 0.0f, 1.0f);
+        s.q = fmaf(hw, (u * u) * fmaf(u, u, u), s.q);
+    }
+    // soft-core Lennard-Jones (:62-71), times c (c = lj0, or 1 where
+    // lj0 = 0): with X = c pp1, c (pp1 - pp2) / pp4 =
+    // (X - c p2a r^6) / (1 + X (lj0 / c) (1 + lj1 d2))
+    const float r2 = rinv * rinv;
+    const float r3 = r2 * rinv;
+    const float r6 = r3 * r3;
+    const float X = q.p1a * (r6 * r3);
+    const float pp4 = fmaf(X, fmaf(sc.lj_b, d2, sc.lj_a), 1.0f);
+    s.v = fmaf(fmaf(-q.p2a, r6, X), __builtin_amdgcn_rcpf(pp4), s.v);
+    s.e = fmaf(ele, rinv, s.e);
+    // contact potential pmf1 / (1 + exp((6 - d/2)(d - pmf0))) (:95-98)
+    const float xe = fmaf(sc.pmf_a, d2, fmaf(q.pmfb, d, q.pmfc));
+    s.p = fmaf(q.pmf1, __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(xe)), s.p);
+    // hydrogen bond hdb1 exp(-((d - hdb0) hdb1)^2 / 2) (:138-143)
+    if (HDB) {
+        const float xh = fmaf(q.hk2, d2, fmaf(q.hk1, d, q.hk0));
+        s.h = fmaf(q.hdb1, __builtin_amdgcn_exp2f(xh), s.h);
+    }
+}
 
 // Pair terms of one atom against points [pb, pe) of one type segment
 // (src/backend_cpu_mic/energy_prt_v1.cpp:30-146) except the pocket-specific
 // term, which the job's short PSP point list adds separately.  HPC: every
 // point of the segment is a hydrophobic-contact point (cdc with nonzero hpp);
 // the other points live in other segments, so the loop has no per-point
-// branch.
-template <bool HPC>
-__device__ __forceinline__ void pair_block(const float4 *__restrict__ ptc, const float2 *__restrict__ pa, int pb,
-                                           int pe, const Slot &me, const PairParams &q, const DevScalars &sc, Sums &s)
+// branch.  HDB: some lane of the wave has an H-bond with this point type;
+// otherwise every lane's hdb1 is 0 and the term is 0 exactly.  Points go in
+// groups of four with one wave-wide test for electrostatic-core pairs
+// (closer than 1 / el1: clashes, rare), whose corrections are skipped
+// otherwise.
+template <bool HPC, bool HDB>
+__device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_, int pb, int pe, const Slot &me,
+                                           const PairParams &q, const DevScalars &sc, Sums &s)
 {
-#pragma unroll 2
-    for (int p = pb; p < pe; ++p) {
-        const float4 P4 = ptc[p];
+    const cfloat *ptc = (const cfloat *)ptc_;
+    const cfloat *pa = (const cfloat *)pa_; // {weight, type} pairs
+    int p = pb;
+    for (; p + 4 <= pe; p += 4) {
+        float4 P[4];
+        float d2[4], rinv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            P[j] = ld_point(ptc, p + j);
+            const float dx = me.x - P[j].x, dy = me.y - P[j].y, dz = me.z - P[j].z;
+            d2[j] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            rinv[j] = __builtin_amdgcn_rsqf(d2[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            pair_terms<HPC, HDB>(d2[j], rinv[j], P[j].w, HPC ? pa[2 * (p + j)] : 0.0f, q, sc, s);
+        // core corrections where some lane of the wave has a core pair in the group
+        const bool near = (d2[0] < sc.el_dc2) | (d2[1] < sc.el_dc2) | (d2[2] < sc.el_dc2) | (d2[3] < sc.el_dc2);
+        if (__any(near)) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s.e += ele_core_delta(d2[j], rinv[j], P[j].w, sc);
+        }
+    }
+    for (; p < pe; ++p) {
+        const float4 P4 = ld_point(ptc, p);
         const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float rinv = __builtin_amdgcn_rsqf(d2);
-        const float d = d2 * rinv;
-        // hydrophobic contact inside 9 A (:44-50)
-        if (HPC) {
-            const float u3 = fmaf(-d2, 0.5f / 81.0f / 81.0f / 81.0f / 81.0f, 2.5f / 81.0f / 81.0f / 81.0f);
-            const float u2 = fmaf(-d2, u3, 4.5f / 81.0f / 81.0f);
-            const float u1 = fmaf(-d2, u2, 3.5f / 81.0f);
-            const float poly = fmaf(-d2, u1, 1.0f);
-            s.q = fmaf(pa[p].x, d2 <= 81.0f ? poly : 0.0f, s.q);
-        }
-        // soft-core Lennard-Jones (:62-71)
-        const float r2 = rinv * rinv;
-        const float r3 = r2 * rinv;
-        const float r6 = r3 * r3;
-        const float pp1 = q.p1a * (r6 * r3);
-        const float pp2 = q.p2a * r6;
-        const float pp4 = fmaf(pp1 * sc.lj0, fmaf(sc.lj1, d2, 1.0f), 1.0f);
-        s.v = fmaf(pp1 - pp2, __builtin_amdgcn_rcpf(pp4), s.v);
-        // electrostatics with its cubic core (:79-85)
-        const float s1e = sc.el1 * d;
-        const float gpoly = fmaf(s1e * s1e, fmaf(sc.b1, s1e, sc.a1), sc.el0);
-        const float g1 = s1e < 1.0f ? gpoly : rinv * sc.inv_el1;
-        s.e = fmaf(P4.w, g1, s.e);
-        // contact potential pmf1 / (1 + exp((6 - d/2)(d - pmf0))) (:95-98)
-        const float dm = d - q.pmf0;
-        const float xe = fmaf(-0.5f * kLog2e, d, 6.0f * kLog2e) * dm;
-        s.p = fmaf(q.pmf1, __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(xe)), s.p);
-        // hydrogen bond hdb1 exp(-((d - hdb0) hdb1)^2 / 2), hdb1 == 0 where
-        // the pair has none (:138-143); hdb1s = hdb1 sqrt(log2(e) / 2)
-        const float h3 = (d - q.hdb0) * q.hdb1s;
-        s.h = fmaf(q.hdb1, __builtin_amdgcn_exp2f(-h3 * h3), s.h);
+        pair_terms<HPC, HDB>(d2, rinv, P4.w, HPC ? pa[2 * p] : 0.0f, q, sc, s);
+        s.e += ele_core_delta(d2, rinv, P4.w, sc);
     }
 }
 
@@ -270,17 +321,26 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
-            const PairParams q{pA.x, pA.y, pA.z, pA.w, pB.x, pB.y, pB.z};
+            const float4 pC = *(const float4 *)(P + 8);
+            const PairParams q{pA.x, pA.y, pA.w, pB.x, pB.y, pB.z, pB.w, pC.x, pC.y};
+            const bool hdb = __any(q.hdb1 != 0.0f); // wave-uniform
             // plain FP32 sums over blocks of at most kBlock points, block sums
             // added with compensation: the electrostatic sum cancels by up to
             // 1e4 over a pose, which long running FP32 sums cannot hold
             for (int pb = b; pb < e; pb += kBlock) {
                 const int pe = min(pb + kBlock, e);
                 Sums bs{0.f, 0.f, 0.f, 0.f, 0.f};
-                if (sg.w) // segment flag: hydrophobic-contact points
-                    pair_block<true>(ptc, tg.pa, pb, pe, me, q, sc, bs);
-                else
-                    pair_block<false>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                if (sg.w) { // segment flag: hydrophobic-contact points
+                    if (hdb)
+                        pair_block<true, true>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                    else
+                        pair_block<true, false>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                } else {
+                    if (hdb)
+                        pair_block<false, true>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                    else
+                        pair_block<false, false>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                }
                 acc_add<KAHAN>(evdw, cvdw, bs.v);
                 acc_add<KAHAN>(eele, cele, bs.e);
                 acc_add<KAHAN>(epmf, cpmf, bs.p);
@@ -364,8 +424,8 @@ __global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, c
                 mcs += J.mtcc[row] * sqrtf(s / (float)(re - rb));
             }
             float *v = vals + (size_t)slot * 8;
-            v[0] = evdw;
-            v[1] = lc * eele;
+            v[0] = evdw * sc.inv_lj0;
+            v[1] = lc * (eele * sc.inv_el1);
             v[2] = epmf;
             v[3] = epsp;
             v[4] = ehdb;
