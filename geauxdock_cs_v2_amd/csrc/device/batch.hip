@@ -104,7 +104,10 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
     }
     std::vector<Shape> cands;
     const int block = j.n_tmp * j.n_lig;
-    for (int W : {4, 8, 16})
+    // workgroups of 4-16 waves; 6 and 12 allow three slot waves (192 lanes,
+    // which ligands of 48, 64 or 96 atoms fill without idle lanes), and two
+    // 12-wave groups fill a CU at the kernel's 6 waves per SIMD
+    for (int W : {4, 6, 8, 12, 16})
         for (int SW = 1; SW <= W; ++SW) {
             if (W % SW || 64 * SW < j.natom) continue;
             const Shape s = packed_shape(j.natom, W, SW, block);
@@ -120,14 +123,18 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
     hipEvent_t e0, e1;
     GD_HIP(hipEventCreate(&e0));
     GD_HIP(hipEventCreate(&e1));
-    const int trial = 2;
+    // two timed steps per candidate; one, untimed warm-up dropped, once a
+    // step holds more than ~1e11 pair evaluations (a few 100 ms)
+    const double pairs = (double)j.n_rep * j.natom * t.npoint;
+    const bool large = pairs > 1e11;
+    const int trial = large ? 1 : 2;
     float best_ms = 1e30f;
     Shape best = cands[0];
     for (const Shape &s : cands) {
         std::vector<DevTask> tasks;
         build_tasks(j, 0, s, s1, trial, tasks);
         // warm the code object and caches once, then time
-        for (int rep = 0; rep < 2; ++rep) {
+        for (int rep = large ? 1 : 0; rep < 2; ++rep) {
             GD_HIP(hipEventRecord(e0, ctx->stream));
             if (int rc = run_tasks(ctx, t, j.self.as<DevJob>(), tasks, s.waves, s.nrep, compensated, false)) return rc;
             GD_HIP(hipEventRecord(e1, ctx->stream));
@@ -194,7 +201,7 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
         if (int rc = autotune_shape(ctx, t, *js[0], 0, comp, shapes[0])) return rc;
         W = shapes[0].waves;
     } else {
-        if (W <= 0) W = 8;
+        if (W <= 0) W = 12;
         for (int i = 0; i < n_jobs; ++i) {
             if (js[i]->natom > 64 * W) return fail(GD_ERR_ARG, "ligand too large for the workgroup");
             shapes[i] = heuristic_shape(js[i]->natom, W, js[i]->n_tmp * js[i]->n_lig);

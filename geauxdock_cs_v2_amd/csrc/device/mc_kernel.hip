@@ -203,8 +203,10 @@ size_t mc_lds_bytes(int waves, int nrep)
     return sizeof(float) * ((size_t)lanes * 8 + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * 4;
 }
 
+// At least 6 waves per SIMD (at most 80 VGPRs): two 12-wave workgroups share
+// a CU.  The spills this costs are outside the pair loops.
 template <bool KAHAN, int MODE>
-__global__ void __launch_bounds__(1024) mc_kernel(DevTarget tg, DevScalars sc, const DevJob *__restrict__ jobs,
+__global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc, const DevJob *__restrict__ jobs,
                                                   const DevTask *__restrict__ tasks,
                                                   const int *__restrict__ step_base, int keep_records,
                                                   const int *__restrict__ pose_rep,

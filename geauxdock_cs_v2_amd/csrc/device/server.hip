@@ -33,7 +33,7 @@
 namespace gd {
 namespace {
 
-constexpr int kServerWaves = 8;
+constexpr int kServerWaves = 12;
 
 // host copy of the job tables of a submitted complex
 struct JobCopy {
