@@ -51,22 +51,35 @@ static Shape packed_shape(int natom, int W, int SW, int max_rep_block)
     return s;
 }
 
-// deterministic choice for a given W: best lane utilisation, then fewest slot waves
-Shape heuristic_shape(int natom, int W, int max_rep_block)
+// Deterministic choice for a given W.  Score = lane use x slot fill: lane
+// use counts the lanes the job's replicas occupy over every workgroup it
+// launches (partly filled last groups included); slot fill is
+// tasks / (slots * rounds), rounds = ceil(tasks / slots), for `slots`
+// workgroups resident at once.  Within 2% of the best score, the most slot
+// waves win (fewest point slices: least cross-slice work per step).
+Shape heuristic_shape(int natom, int W, int max_rep_block, int n_prt, int slots)
 {
-    Shape best{0, 0, 0, 0};
-    double best_u = -1.0;
+    slots = std::max(1, slots);
+    Shape best = packed_shape(natom, W, W, max_rep_block); // natom > 64 W: caller rejects
+    std::vector<std::pair<Shape, double>> cs;
+    double top = -1.0;
     for (int SW = 1; SW <= W; ++SW) {
-        if (W % SW) continue;
-        if (64 * SW < natom) continue;
+        if (W % SW || 64 * SW < natom) continue;
         const Shape s = packed_shape(natom, W, SW, max_rep_block);
-        const double u = (double)s.nrep * natom / (64.0 * SW);
-        if (u > best_u + 1e-9) {
-            best_u = u;
-            best = s;
-        }
+        const long per_prt = (max_rep_block + s.nrep - 1) / s.nrep;
+        const long tasks = (long)n_prt * per_prt;
+        const double use = (double)max_rep_block * natom / ((double)per_prt * 64.0 * SW);
+        const long rounds = (tasks + slots - 1) / slots;
+        const double fill = (double)tasks / ((double)rounds * slots);
+        cs.emplace_back(s, use * fill);
+        top = std::max(top, use * fill);
     }
-    if (best.waves == 0) best = packed_shape(natom, W, W, max_rep_block); // natom > 64 W: caller rejects
+    int best_sw = -1;
+    for (const auto &c : cs)
+        if (c.second >= top * 0.98 && c.first.slot_waves > best_sw) {
+            best_sw = c.first.slot_waves;
+            best = c.first;
+        }
     return best;
 }
 
@@ -202,9 +215,11 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
         W = shapes[0].waves;
     } else {
         if (W <= 0) W = 12;
+        // each job's share of the workgroups resident at once (two per CU)
+        const int slots = std::max(1, 2 * std::max(ctx->cu_count, 1) / n_jobs);
         for (int i = 0; i < n_jobs; ++i) {
             if (js[i]->natom > 64 * W) return fail(GD_ERR_ARG, "ligand too large for the workgroup");
-            shapes[i] = heuristic_shape(js[i]->natom, W, js[i]->n_tmp * js[i]->n_lig);
+            shapes[i] = heuristic_shape(js[i]->natom, W, js[i]->n_tmp * js[i]->n_lig, js[i]->n_prt, slots);
         }
     }
     int max_rep = 1;

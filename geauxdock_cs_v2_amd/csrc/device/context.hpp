@@ -70,8 +70,9 @@ struct Shape {
 };
 
 Shape plain_shape(int natom);
-// replica packing for a W-wave workgroup: best lane use, then fewest slot waves
-Shape heuristic_shape(int natom, int W, int max_rep_block);
+// replica packing for a W-wave workgroup: best lane use x fill of `slots`
+// resident workgroups, then the fewest point slices
+Shape heuristic_shape(int natom, int W, int max_rep_block, int n_prt, int slots);
 // tasks of one job: consecutive replicas of each protein conformation, R per task
 void build_tasks(const JobRes &j, int job_slot, const Shape &sh, int s1, int s2max, std::vector<DevTask> &out);
 

@@ -34,6 +34,8 @@ namespace gd {
 namespace {
 
 constexpr int kServerWaves = 12;
+// a request's shape depends on the request alone (so its result does not
+// depend on the mix): the shape that would fill the GPU best were it alone
 
 // host copy of the job tables of a submitted complex
 struct JobCopy {
@@ -186,7 +188,7 @@ void gd_server::loop()
                 JobRes &j = ctx->jobs.at(q.job_id);
                 table.push_back(j.hj);
                 const int s2 = std::min(opts.chunk_steps, q.steps - q.done);
-                build_tasks(j, (int)i, heuristic_shape(j.natom, kServerWaves, j.n_tmp * j.n_lig), q.done, s2, tasks);
+                build_tasks(j, (int)i, heuristic_shape(j.natom, kServerWaves, j.n_tmp * j.n_lig, j.n_prt, 2 * ctx->cu_count), q.done, s2, tasks);
             }
         }
         int max_rep = 1;

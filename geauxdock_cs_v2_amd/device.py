@@ -92,6 +92,44 @@ class Device:
         return ms.value
 
 
+class Server:
+    """The serving loop on a Device's resident target (gd_server_*): submit
+    staged complexes, poll or drain, read timings on the server clock.  The
+    server owns the device context until close()."""
+
+    def __init__(self, dev, target, hbm_budget, max_active=64, chunk=100, exchange=0):
+        self.lib = dev.lib
+        self.h = C.c_void_p()
+        o = D.ServerOpts(int(hbm_budget), int(max_active), int(chunk), int(exchange))
+        L.check(self.lib.gd_server_create(dev.h, target, C.byref(o), C.byref(self.h)))
+        self.n_rep = {}
+
+    def submit(self, cx, steps, seed):
+        i = C.c_int()
+        L.check(self.lib.gd_server_submit(self.h, C.byref(cx.view), int(steps), int(seed), C.byref(i)))
+        self.n_rep[i.value] = cx.size["n_rep"]
+        return i.value
+
+    def clock_ms(self):
+        return self.lib.gd_server_clock_ms(self.h)
+
+    def drain(self):
+        L.check(self.lib.gd_server_drain(self.h))
+
+    def poll(self, req, replicas=False):
+        """(RequestResult, final replicas or None); result.status is
+        GD_ERR_BUSY while the request is pending, 0 once done."""
+        res = D.RequestResult()
+        reps = (D.ReplicaMC * self.n_rep[req])() if replicas else None
+        L.check(self.lib.gd_server_poll(self.h, req, C.byref(res), reps))
+        return res, reps
+
+    def close(self):
+        if self.h:
+            self.lib.gd_server_destroy(self.h)
+            self.h = None
+
+
 class Docker:
     """The reference's backend interface (src/backend_gpu/backend_gpu.h:10-23):
     a staged `complex`, the `record` window of the last chunk, and run(),

@@ -309,3 +309,91 @@ def load_config(name, seed=2024, lib=None, **override):
     raw = L.load_job(info["pdb"], info["sdf"], info["ff"], info["lig_id"], n_tmp=c["n_tmp"],
                      temp_high=c["t_hi"], temp_low=c["t_lo"], lib=lib)
     return raw.stage(), info
+
+
+# ---------------------------------------------------------------------------
+# many-ligand workloads: batched screening and the serving request stream
+# ---------------------------------------------------------------------------
+# screening: one resident 5,000-point protein in 6 conformations; 1,024
+# ligands whose heavy-atom counts are lognormal (median 32, sigma 0.5)
+# clipped to 12-96, 8 conformations each, 8 temperatures: ~393k replicas
+SCREEN = dict(n_ligands=1024, median_atoms=32, log_sigma=0.5, min_atoms=12, max_atoms=96, n_lig_conf=8,
+              lig_sigma=0.4, n_points=5000, n_prt_conf=6, prt_sigma=0.5, prt_radius=30.0, n_tmp=8, t_lo=0.3,
+              t_hi=2.0, steps=2000)
+# serving: 8,000-point protein in 4 conformations; requests of 10-120 atoms
+# (uniform), 4-32 conformations, 8 temperatures, 500-5,000 steps
+# (log-uniform), Poisson arrivals at 50, 200 and 800 per second
+SERVE = dict(n_points=8000, n_prt_conf=4, prt_sigma=0.5, prt_radius=35.0, atoms=(10, 120), confs=(4, 32),
+             n_tmp=8, t_lo=0.3, t_hi=2.0, steps=(500, 5000), rates=(50.0, 200.0, 800.0), lig_sigma=0.4)
+
+
+def _ligand_files(d, lig_id, rng_seed, n_atoms, n_conf, lig_sigma, protein):
+    """One ligand (.sdf/.ff) of a many-ligand workload against `protein`;
+    KDE 8 points per atom, MCS 8 rows of up to 12 columns."""
+    return generate(d, lig_id, seed=rng_seed, n_atoms=n_atoms, n_lig_conf=n_conf, lig_sigma=lig_sigma,
+                    n_points=0, n_prt_conf=0, prt_sigma=0.0, prt_radius=0.0, n_kde=8 * n_atoms, mcs_rows=8,
+                    mcs_cols=12, protein=protein)
+
+
+def _protein_only(d, name, seed, n_points, n_conf, sigma, radius):
+    rng = np.random.default_rng(seed)
+    codes, sites = make_protein(rng, n_points, n_conf, sigma, radius)
+    pdb = os.path.join(d, f"{name}_prt.pdb")
+    _, _, sc_idx = write_protein(pdb, codes, sites)
+    return {"pdb": pdb, "codes": codes, "sites": sites[0], "sc_idx": sc_idx}
+
+
+def many_files(kind, seed=2024, outdir=None, n=None):
+    """Files of the screening library (kind "screen") or of the serving
+    request stream (kind "serve"), generated once and cached by content;
+    returns {"pdb", "ligands": [{"sdf", "ff", "lig_id", "n_atoms", "n_conf",
+    "steps"?}], "cfg"}."""
+    cfg = dict(SCREEN if kind == "screen" else SERVE)
+    if n is not None:
+        cfg["n_ligands"] = n
+    key = hashlib.sha1(json.dumps([kind, seed, cfg], sort_keys=True).encode()).hexdigest()[:12]
+    d = os.path.join(outdir or default_dir(), f"{kind}_{key}")
+    meta = os.path.join(d, "meta.json")
+    if os.path.exists(meta):
+        with open(meta) as f:
+            return json.load(f)
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    prot = _protein_only(d, kind, seed + 1, cfg["n_points"], cfg["n_prt_conf"], cfg["prt_sigma"], cfg["prt_radius"])
+    ligs = []
+    count = cfg.get("n_ligands", 0) if kind == "screen" else int(n or 0)
+    for i in range(count):
+        if kind == "screen":
+            na = int(np.clip(round(cfg["median_atoms"] * math.exp(cfg["log_sigma"] * rng.normal())),
+                             cfg["min_atoms"], cfg["max_atoms"]))
+            nc = cfg["n_lig_conf"]
+            steps = cfg["steps"]
+        else:
+            na = int(rng.integers(cfg["atoms"][0], cfg["atoms"][1] + 1))
+            nc = int(rng.integers(cfg["confs"][0], cfg["confs"][1] + 1))
+            lo, hi = math.log(cfg["steps"][0]), math.log(cfg["steps"][1])
+            steps = int(round(math.exp(rng.uniform(lo, hi))))
+        lig_id = f"{kind}{i:04d}"
+        out = _ligand_files(d, lig_id, seed * 7919 + i, na, nc, cfg["lig_sigma"], prot)
+        ligs.append({"sdf": out["sdf"], "ff": out["ff"], "lig_id": lig_id, "n_atoms": na, "n_conf": nc,
+                     "steps": steps})
+    info = {"pdb": prot["pdb"], "ligands": ligs, "cfg": cfg, "seed": seed}
+    with open(meta, "w") as f:
+        json.dump(info, f)
+    return info
+
+
+def load_many(info, threads=16, lib=None):
+    """Load and stage every ligand of many_files() against its protein, in
+    parallel (the loader releases the interpreter while it parses)."""
+    import concurrent.futures as cf
+    from . import lib as L
+    c = info["cfg"]
+
+    def one(lg):
+        raw = L.load_job(info["pdb"], lg["sdf"], lg["ff"], lg["lig_id"], n_tmp=c["n_tmp"], temp_high=c["t_hi"],
+                         temp_low=c["t_lo"], lib=lib)
+        return raw.stage()
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, info["ligands"]))
