@@ -281,22 +281,23 @@ def main():
               "(replica ladders shard by job across GPUs with no collective in the MC loop)", file=sys.stderr)
         sys.exit(2)
     sections = set(s.strip() for s in args.sections.split(",") if s.strip())
-    if "c2" not in sections:
-        sections.add("c2")  # the headline value
+    # without c2 (profiling runs of one section) the line has no headline value
 
     from geauxdock_cs_v2_amd import perf, synth
     from geauxdock_cs_v2_amd.device import Device, Server
 
     dev = Device(0)
-    cx, c2 = run_c2(args, dev, perf, synth)
+    cx, c2 = run_c2(args, dev, perf, synth) if "c2" in sections else (None, None)
     screen = run_screen(args, dev, perf, synth) if "screen" in sections else None
     serve = run_serve(args, dev, perf, synth, Server) if "serve" in sections else None
     dev.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(cx, args.cpu_seconds)
+    cpu = None if (args.no_cpu or cx is None) else cpu_baseline(cx, args.cpu_seconds)
     if cpu:
         log(f"cpu oracle: {cpu['value']:.4g} poses/s ({cpu['sample']})")
 
+    c2 = c2 or {"value": None, "wall": float("nan"), "workload": None, "n_rep": None, "roofline": None,
+                "plain": {"value": None}}
     value = c2["value"]
     plain = c2["plain"]
     out = {
@@ -306,7 +307,7 @@ def main():
         "config": {"workload": c2["workload"], "replicas": c2["n_rep"],
                    "form": "batched (packed replicas, autotuned shape, hipGraph chunks)"},
         "plain": plain,
-        "speedup_vs_plain": value / plain["value"] if plain["value"] else None,
+        "speedup_vs_plain": value / plain["value"] if (value and plain["value"]) else None,
         "roofline": c2["roofline"],
         "screen": screen,
         "serve": serve,

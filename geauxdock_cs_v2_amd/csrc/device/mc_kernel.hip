@@ -204,11 +204,14 @@ size_t mc_lds_bytes(int waves, int nrep)
 }
 
 // At least 6 waves per SIMD (at most 80 VGPRs): two 12-wave workgroups share
-// a CU.  The spills this costs are outside the pair loops.
-template <bool KAHAN, int MODE>
+// a CU.  The spills this costs are outside the pair loops.  RECORDS: append
+// every accepted step to the job's record window (the plain chunk keeps
+// them; batched runs usually do not, and the instantiations have distinct
+// names in profiles).
+template <bool KAHAN, int MODE, bool RECORDS>
 __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc, const DevJob *__restrict__ jobs,
                                                   const DevTask *__restrict__ tasks,
-                                                  const int *__restrict__ step_base, int keep_records,
+                                                  const int *__restrict__ step_base,
                                                   const int *__restrict__ pose_rep,
                                                   const float *__restrict__ pose_move, float *__restrict__ pose_out)
 {
@@ -488,7 +491,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                     for (int k = 0; k < 6; ++k) cur[k] = pm[k];
                     for (int k = 0; k < GD_MAXWEI; ++k) cur[6 + k] = e[k];
                     ist[gg * 4 + 2] = stp;
-                    if (keep_records) {
+                    if (RECORDS) {
                         const int r = task.rep0 + gg;
                         const int ne = ist[gg * 4 + 0];
                         gd_ReplicaMC *rc = J.rec + (size_t)r * J.rec_cap + ne;
@@ -516,7 +519,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             for (int k = 0; k < GD_MAXWEI; ++k) rp.energy[k] = st[i * 16 + 6 + k];
             rp.step = ist[i * 4 + 2];
             rp.is_accept = ist[i * 4 + 1];
-            if (keep_records) J.next_entry[task.rep0 + i] = ist[i * 4 + 0];
+            if (RECORDS) J.next_entry[task.rep0 + i] = ist[i * 4 + 0];
         }
     }
 }
@@ -563,12 +566,18 @@ hipError_t launch_mc(const DevTarget &tg, const DevScalars &sc, const DevJob *jo
     if (ntask <= 0) return hipSuccess;
     const size_t lds = mc_lds_bytes(waves, max_rep);
     const dim3 grid(ntask), block(waves * 64);
-    if (compensated)
-        hipLaunchKernelGGL((mc_kernel<true, MODE_MC>), grid, block, lds, stream, tg, sc, jobs, tasks, step_base,
-                           keep_records ? 1 : 0, nullptr, nullptr, nullptr);
+    if (compensated && keep_records)
+        hipLaunchKernelGGL((mc_kernel<true, MODE_MC, true>), grid, block, lds, stream, tg, sc, jobs, tasks, step_base,
+                           nullptr, nullptr, nullptr);
+    else if (compensated)
+        hipLaunchKernelGGL((mc_kernel<true, MODE_MC, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
+                           step_base, nullptr, nullptr, nullptr);
+    else if (keep_records)
+        hipLaunchKernelGGL((mc_kernel<false, MODE_MC, true>), grid, block, lds, stream, tg, sc, jobs, tasks,
+                           step_base, nullptr, nullptr, nullptr);
     else
-        hipLaunchKernelGGL((mc_kernel<false, MODE_MC>), grid, block, lds, stream, tg, sc, jobs, tasks, step_base,
-                           keep_records ? 1 : 0, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((mc_kernel<false, MODE_MC, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
+                           step_base, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -580,11 +589,11 @@ hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob
     const size_t lds = mc_lds_bytes(waves, 1);
     const dim3 grid(ntask), block(waves * 64);
     if (compensated)
-        hipLaunchKernelGGL((mc_kernel<true, MODE_ENERGY>), grid, block, lds, stream, tg, sc, jobs, tasks, nullptr, 0,
-                           pose_rep, pose_move, pose_out);
+        hipLaunchKernelGGL((mc_kernel<true, MODE_ENERGY, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
+                           nullptr, pose_rep, pose_move, pose_out);
     else
-        hipLaunchKernelGGL((mc_kernel<false, MODE_ENERGY>), grid, block, lds, stream, tg, sc, jobs, tasks, nullptr,
-                           0, pose_rep, pose_move, pose_out);
+        hipLaunchKernelGGL((mc_kernel<false, MODE_ENERGY, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
+                           nullptr, pose_rep, pose_move, pose_out);
     return hipGetLastError();
 }
 
