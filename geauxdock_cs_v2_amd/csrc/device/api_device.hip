@@ -176,20 +176,32 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     const float origin[3] = {cx->prt_pocket_center[0], cx->prt_pocket_center[1], cx->prt_pocket_center[2]};
 
     // per point: hydrophobic-contact weight (cdc ? hpp : 0); points are
-    // re-sorted by (type, hydrophobic flag), keeping the class order
-    // otherwise, so a segment shares one set of type-pair parameters and one
-    // term set
+    // re-sorted by (type, hydrophobic flag), so a segment shares one set of
+    // type-pair parameters and one term set
     std::vector<float> hpcw(np);
     std::vector<int> flags(np);
     for (int s = 0; s < np; ++s) {
         hpcw[s] = cx->prt_cdc[s] == 1 ? cx->prt_hpp[s] : 0.0f;
         flags[s] = hpcw[s] != 0.0f ? 1 : 0;
     }
+    // smallest distance of each point from the frame origin over the
+    // conformations (in the float coordinates the kernel reads): within a
+    // segment points go nearest first, so the points beyond any radius are a
+    // suffix of the segment
+    std::vector<double> rmin(np, 1e300);
+    for (int c = 0; c < nconf; ++c)
+        for (int i = 0; i < np; ++i) {
+            const size_t k = (size_t)c * np + i;
+            const double x = local(cx->prt_x[k], origin[0]), y = local(cx->prt_y[k], origin[1]),
+                         z = local(cx->prt_z[k], origin[2]);
+            rmin[i] = std::min(rmin[i], std::sqrt(x * x + y * y + z * z));
+        }
     std::vector<int> ord(np);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
         if (cx->prt_t[x] != cx->prt_t[y]) return cx->prt_t[x] < cx->prt_t[y];
-        return flags[x] > flags[y];
+        if (flags[x] != flags[y]) return flags[x] > flags[y];
+        return rmin[x] < rmin[y];
     });
     std::vector<float2> pa(np);
     for (int i = 0; i < np; ++i) {
@@ -238,12 +250,48 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             q[PP_HDB0] = ep.hdb0[tl][tp];
         }
 
+    // Far field.  Beyond far_d every pair term but LJ and 1/d electrostatics
+    // is constant to float precision: the hydrophobic cutoff (9 A) and the
+    // electrostatic core (1/el1) are behind; the contact sigmoid's exp factor
+    // is below 2^-26, so pmf1 / (1 + e) rounds to pmf1; the H-bond Gaussian is
+    // below e^-30 of hdb1.  A point whose smallest radius is r is at least
+    // r - rho from every atom within rho of the origin, so per segment the
+    // points at r >= rho + far_d are far for all of them; ladder[seg][k] is
+    // the first such point for rho + far_d <= kFarR0 + k * kFarDr.
+    double far_d = std::max(9.0, 1.0 / (double)ep.el1);
+    {
+        const double tp_ = 26.0 * std::log(2.0);
+        for (int tp = 0; tp < GD_MAXTP1; ++tp)
+            for (int tl = 0; tl < GD_MAXTP2; ++tl) {
+                if (ep.pmf1[tl][tp] != 0.0f) {
+                    // (d/2 - 6)(d - pmf0) >= tp_, d > 12
+                    const double p0 = ep.pmf0[tl][tp], bq = 6.0 + 0.5 * p0;
+                    const double disc = bq * bq - 2.0 * (6.0 * p0 - tp_);
+                    far_d = std::max(far_d, bq + std::sqrt(std::max(disc, 0.0)));
+                }
+                if (ep.hdb0[tl][tp] > 0.1f && ep.hdb1[tl][tp] != 0.0f) // ((d - h0) h1)^2 / 2 >= 30
+                    far_d = std::max(far_d, (double)ep.hdb0[tl][tp] + std::sqrt(60.0) / std::fabs((double)ep.hdb1[tl][tp]));
+            }
+        far_d += 0.05;
+    }
+    std::vector<int> ladder(seg.size() * kFarRungs);
+    for (size_t g = 0; g < seg.size(); ++g)
+        for (int k = 0; k < kFarRungs; ++k) {
+            const double rk = kFarR0 + k * kFarDr;
+            int i = seg[g].y;
+            while (i < seg[g].z && rmin[ord[i]] < rk) ++i;
+            ladder[g * kFarRungs + k] = i;
+        }
+
     const int id = ctx->next_id++;
     TargetRes &t = ctx->targets[id];
     GD_HIP(upload(t.pt, pt.data(), sizeof(float4) * pt.size(), ctx->stream));
     GD_HIP(upload(t.pa, pa.data(), sizeof(float2) * pa.size(), ctx->stream));
     GD_HIP(upload(t.pp, pp.data(), sizeof(float) * pp.size(), ctx->stream));
     GD_HIP(upload(t.seg, seg.data(), sizeof(int4) * seg.size(), ctx->stream));
+    GD_HIP(upload(t.ladder, ladder.data(), sizeof(int) * ladder.size(), ctx->stream));
+    t.dt.ladder = t.ladder.as<int>();
+    t.dt.far_d = (float)far_d;
     t.dt.pt = t.pt.as<float4>();
     t.dt.pa = t.pa.as<float2>();
     t.dt.pp = t.pp.as<float>();
@@ -285,7 +333,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         t.seg_host.push_back(g.y);
         t.seg_host.push_back(g.z);
     }
-    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.seg.bytes;
+    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes;
     t.dev_index.assign(np, 0);
     for (int i = 0; i < np; ++i) t.dev_index[ord[i]] = i;
     t.prt_t.assign(cx->prt_t, cx->prt_t + np);

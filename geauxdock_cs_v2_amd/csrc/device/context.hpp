@@ -39,7 +39,7 @@ struct DevBuf {
 };
 
 struct TargetRes {
-    DevBuf pt, pa, pp, seg;
+    DevBuf pt, pa, pp, seg, ladder;
     DevTarget dt{};
     DevScalars sc{};
     int n_prt = 0, npoint = 0;

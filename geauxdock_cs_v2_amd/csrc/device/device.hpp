@@ -8,7 +8,11 @@
 //                                     flag) so a run of points shares one set
 //                                     of type-pair parameters and terms
 //           pa   float2[npad]         {hpc weight = (cdc ? hpp : 0) / 2, point type}
-//           seg  int4[nseg]           {point type, begin, end, hydrophobic flag}
+//           seg  int4[nseg]           {point type, begin, end, hydrophobic flag};
+//                                     within a segment points go by their
+//                                     smallest radius over the conformations
+//           ladder int[nseg][kFarRungs] first point of each segment at or
+//                                     beyond each radius rung (far field)
 //           pp   float[31][24][PP_N]  type-pair parameters, see PairParam
 //   job     lig3 float4[nlig][natom]  {x3, y3, z3, c3} type-sorted atoms
 //           lig2 float4[nlig][natom]  {x2, y2, z2, 0} file-order atoms (MCS)
@@ -62,11 +66,18 @@ struct DevScalars {
     float a_para[GD_MAXWEI], b_para[GD_MAXWEI], w[GD_MAXWEI];
 };
 
+// Far-field radius ladder: rung k stands for radius kFarR0 + k * kFarDr.
+constexpr int kFarRungs = 128;
+constexpr float kFarR0 = 8.0f;
+constexpr float kFarDr = 0.5f;
+
 struct DevTarget {
     const float4 *pt;  // [nconf][npad] {x, y, z, ele}, pocket-centred frame
     const float2 *pa;  // [npad] {hydrophobic weight / 2, point type (int bits)}
     const float *pp;   // [31][24][PP_N] type-pair parameters
     const int4 *seg;   // [nseg] {point type, begin, end, hydrophobic flag}
+    const int *ladder; // [nseg][kFarRungs] first point of the segment at radius >= rung
+    float far_d;       // distance beyond which only LJ and 1/d electrostatics vary
     int npoint, npad, nseg, nconf;
 };
 

@@ -77,6 +77,7 @@ struct Slot {
 };
 
 constexpr int kBlock = 64; // points per plain FP32 partial sum
+constexpr int kVals = 9;   // per-slot terms: vdw, ele hi, pmf, psp, hdb, hpc, kde, mcs, ele lo
 
 } // namespace
 
@@ -88,12 +89,23 @@ struct PairParams {
 
 struct Sums {
     float v, e, p, h, q; // vdw (lj0-scaled), ele (el1-scaled, without the atom charge), pmf, hdb, hydrophobic
+    float ec;            // compensation of e: the sum is e - ec
 };
+
+// Compensated (Kahan) add of v to the pair (s, c), whose value is s - c.
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v)
+{
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
 
 // The target tables are read-only while a kernel runs: reading them through
 // the constant address space lets the compiler fetch wave-uniform points with
 // scalar loads into SGPRs instead of 64 identical vector loads.
 typedef const __attribute__((address_space(4))) float cfloat;
+typedef const __attribute__((address_space(4))) int cint;
 
 __device__ __forceinline__ float4 ld_point(const cfloat *pt, int p)
 {
@@ -111,8 +123,21 @@ __device__ __forceinline__ float ele_core_delta(float d2, float rinv, float ele,
     return d2 < sc.el_dc2 ? ele * (core - rinv) : 0.0f;
 }
 
-// Terms of one pair once d2 and 1/d are known (electrostatics outside the
-// core only).
+// Soft-core Lennard-Jones (:62-71), times c (c = lj0, or 1 where lj0 = 0):
+// with X = c pp1, c (pp1 - pp2) / pp4 = (X - c p2a r^6) / (1 + X (lj0 / c)(1 + lj1 d2)).
+__device__ __forceinline__ void lj_term(float d2, float rinv, const PairParams &q, const DevScalars &sc, Sums &s)
+{
+    const float r2 = rinv * rinv;
+    const float r3 = r2 * rinv;
+    const float r6 = r3 * r3;
+    const float X = q.p1a * (r6 * r3);
+    const float pp4 = fmaf(X, fmaf(sc.lj_b, d2, sc.lj_a), 1.0f);
+    s.v = fmaf(fmaf(-q.p2a, r6, X), __builtin_amdgcn_rcpf(pp4), s.v);
+}
+
+// Terms of one pair once d2 and 1/d are known, electrostatics aside: the
+// electrostatic sum cancels by up to 1e6 over a pose, so its terms are added
+// four at a time with compensation by the caller.
 template <bool HPC, bool HDB>
 __device__ __forceinline__ void pair_terms(float d2, float rinv, float ele, float hw, const PairParams &q,
                                            const DevScalars &sc, Sums &s)
@@ -125,16 +150,7 @@ __device__ __forceinline__ void pair_terms(float d2, float rinv, float ele, floa
         const float u = __builtin_amdgcn_fmed3f(fmaf(-d2, 1.0f / 81.0f, 1.0f), 0.0f, 1.0f);
         s.q = fmaf(hw, (u * u) * fmaf(u, u, u), s.q);
     }
-    // soft-core Lennard-Jones (:62-71), times c (c = lj0, or 1 where
-    // lj0 = 0): with X = c pp1, c (pp1 - pp2) / pp4 =
-    // (X - c p2a r^6) / (1 + X (lj0 / c) (1 + lj1 d2))
-    const float r2 = rinv * rinv;
-    const float r3 = r2 * rinv;
-    const float r6 = r3 * r3;
-    const float X = q.p1a * (r6 * r3);
-    const float pp4 = fmaf(X, fmaf(sc.lj_b, d2, sc.lj_a), 1.0f);
-    s.v = fmaf(fmaf(-q.p2a, r6, X), __builtin_amdgcn_rcpf(pp4), s.v);
-    s.e = fmaf(ele, rinv, s.e);
+    lj_term(d2, rinv, q, sc, s);
     // contact potential pmf1 / (1 + exp((6 - d/2)(d - pmf0))) (:95-98)
     const float xe = fmaf(sc.pmf_a, d2, fmaf(q.pmfb, d, q.pmfc));
     s.p = fmaf(q.pmf1, __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(xe)), s.p);
@@ -172,15 +188,19 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_
             d2[j] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             rinv[j] = __builtin_amdgcn_rsqf(d2[j]);
         }
+        float ge = 0.0f; // the group's electrostatics
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 4; ++j) {
             pair_terms<HPC, HDB>(d2[j], rinv[j], P[j].w, HPC ? pa[2 * (p + j)] : 0.0f, q, sc, s);
+            ge = fmaf(P[j].w, rinv[j], ge);
+        }
         // core corrections where some lane of the wave has a core pair in the group
         const bool near = (d2[0] < sc.el_dc2) | (d2[1] < sc.el_dc2) | (d2[2] < sc.el_dc2) | (d2[3] < sc.el_dc2);
         if (__any(near)) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) s.e += ele_core_delta(d2[j], rinv[j], P[j].w, sc);
+            for (int j = 0; j < 4; ++j) ge += ele_core_delta(d2[j], rinv[j], P[j].w, sc);
         }
+        kahan_add(s.e, s.ec, ge);
     }
     for (; p < pe; ++p) {
         const float4 P4 = ld_point(ptc, p);
@@ -188,8 +208,47 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float rinv = __builtin_amdgcn_rsqf(d2);
         pair_terms<HPC, HDB>(d2, rinv, P4.w, HPC ? pa[2 * p] : 0.0f, q, sc, s);
-        s.e += ele_core_delta(d2, rinv, P4.w, sc);
+        kahan_add(s.e, s.ec, fmaf(P4.w, rinv, ele_core_delta(d2, rinv, P4.w, sc)));
     }
+}
+
+// Far field: points at least far_d from every atom of the wave (see
+// DevTarget::far_d).  Only LJ and 1/d electrostatics vary there; the contact
+// term is pmf1 per point, the hydrophobic, core and H-bond terms are 0.
+__device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int pe, const Slot &me,
+                                               const PairParams &q, const DevScalars &sc, Sums &s)
+{
+    const cfloat *ptc = (const cfloat *)ptc_;
+    int p = pb;
+    for (; p + 4 <= pe; p += 4) {
+        float ge = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 P4 = ld_point(ptc, p + j);
+            const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float rinv = __builtin_amdgcn_rsqf(d2);
+            lj_term(d2, rinv, q, sc, s);
+            ge = fmaf(P4.w, rinv, ge);
+        }
+        kahan_add(s.e, s.ec, ge);
+    }
+    for (; p < pe; ++p) {
+        const float4 P4 = ld_point(ptc, p);
+        const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float rinv = __builtin_amdgcn_rsqf(d2);
+        lj_term(d2, rinv, q, sc, s);
+        kahan_add(s.e, s.ec, P4.w * rinv);
+    }
+    s.p = fmaf(q.pmf1, (float)(pe - pb), s.p);
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
 }
 
 } // namespace
@@ -197,10 +256,11 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_
 size_t mc_lds_bytes(int waves, int nrep)
 {
     const int lanes = waves * 64;
-    // one region for the cross-slice partials and, after a barrier, the
-    // per-slot terms (lanes * 8 floats bounds both); then rsum, st, prop
-    // (16 + 16 + 8 floats per replica) and 4 ints per replica
-    return sizeof(float) * ((size_t)lanes * 8 + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * 4;
+    // one region for the cross-slice partials (7 floats per slot and slice
+    // beyond the first) and, after a barrier, the per-slot terms (kVals
+    // floats per slot); then rsum, st, prop (16 + 16 + 8 floats per replica)
+    // and 4 ints per replica
+    return sizeof(float) * ((size_t)lanes * kVals + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * 4;
 }
 
 // At least 6 waves per SIMD (at most 80 VGPRs): two 12-wave workgroups share
@@ -234,9 +294,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     const int a = active ? slot - g * NL : 0;
     const int lanes = W * 64;
 
-    float *red = lds;                      // [(S-1) * SW*64 * 6], then reused:
-    float *vals = lds;                     // [SW*64 * 8]
-    float *rsum = lds + (size_t)lanes * 8; // [R][16] sums + compensations
+    float *red = lds;                          // [(S-1) * SW*64 * 7], then reused:
+    float *vals = lds;                         // [SW*64 * kVals]
+    float *rsum = lds + (size_t)lanes * kVals; // [R][16] compensated term sums
     float *st = rsum + (size_t)R * 16;     // [R][16] movematrix[6], energy[10]
     float *prop = st + (size_t)R * 16;     // [R][8] proposed move, accept uniform
     int *ist = (int *)(prop + (size_t)R * 8); // [R][4] next_entry, is_accept, step, idx_tmp
@@ -319,10 +379,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // ---- pair terms over this slice's points ----
         float evdw = 0.f, eele = 0.f, epmf = 0.f, epsp = 0.f, ehdb = 0.f, ehpc = 0.f;
         float cvdw = 0.f, cele = 0.f, cpmf = 0.f, cpsp = 0.f, chdb = 0.f, chpc = 0.f;
+        // far field: rho = the wave's largest atom radius; from rung kf of a
+        // segment's ladder on, points are at least far_d from every atom here
+        const float rho = sqrtf(wave_max(fmaf(me.x, me.x, fmaf(me.y, me.y, me.z * me.z))));
+        const int kf = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho + tg.far_d - kFarR0) * (1.0f / kFarDr))));
+        const cint *ladder = (const cint *)tg.ladder;
         for (int si = 0; si < tg.nseg; ++si) {
             const int4 sg = tg.seg[si];
             const int b = max(sg.y, lo), e = min(sg.z, hi);
             if (b >= e) continue;
+            const int fb = kf < kFarRungs ? ladder[si * kFarRungs + kf] : sg.z;
+            const int mid = min(max(fb, b), e); // near [b, mid), far [mid, e)
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
@@ -332,9 +399,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             // plain FP32 sums over blocks of at most kBlock points, block sums
             // added with compensation: the electrostatic sum cancels by up to
             // 1e4 over a pose, which long running FP32 sums cannot hold
-            for (int pb = b; pb < e; pb += kBlock) {
-                const int pe = min(pb + kBlock, e);
-                Sums bs{0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int pb = b; pb < mid; pb += kBlock) {
+                const int pe = min(pb + kBlock, mid);
+                Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (sg.w) { // segment flag: hydrophobic-contact points
                     if (hdb)
                         pair_block<true, true>(ptc, tg.pa, pb, pe, me, q, sc, bs);
@@ -348,9 +415,19 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 }
                 acc_add<KAHAN>(evdw, cvdw, bs.v);
                 acc_add<KAHAN>(eele, cele, bs.e);
+                acc_add<KAHAN>(eele, cele, -bs.ec);
                 acc_add<KAHAN>(epmf, cpmf, bs.p);
                 acc_add<KAHAN>(ehdb, chdb, bs.h);
                 acc_add<KAHAN>(ehpc, chpc, bs.q);
+            }
+            for (int pb = mid; pb < e; pb += kBlock) {
+                const int pe = min(pb + kBlock, e);
+                Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                pair_block_far(ptc, pb, pe, me, q, sc, bs);
+                acc_add<KAHAN>(evdw, cvdw, bs.v);
+                acc_add<KAHAN>(eele, cele, bs.e);
+                acc_add<KAHAN>(eele, cele, -bs.ec);
+                acc_add<KAHAN>(epmf, cpmf, bs.p);
             }
         }
         // pocket-specific term: the job's class-2 points whose residue has a
@@ -374,20 +451,22 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
 
         // ---- cross-slice sum ----
         if (S > 1 && slice > 0) {
-            float *r = red + ((size_t)(slice - 1) * SW * 64 + slot) * 6;
+            float *r = red + ((size_t)(slice - 1) * SW * 64 + slot) * 7;
             r[0] = evdw;
             r[1] = eele;
             r[2] = epmf;
             r[3] = epsp;
             r[4] = ehdb;
             r[5] = ehpc;
+            r[6] = cele; // the electrostatic sum travels with its compensation
         }
         __syncthreads();
         if (slice == 0 && active) {
             for (int s = 1; s < S; ++s) {
-                const float *r = red + ((size_t)(s - 1) * SW * 64 + slot) * 6;
+                const float *r = red + ((size_t)(s - 1) * SW * 64 + slot) * 7;
                 acc_add<KAHAN>(evdw, cvdw, r[0]);
                 acc_add<KAHAN>(eele, cele, r[1]);
+                acc_add<KAHAN>(eele, cele, -r[6]);
                 acc_add<KAHAN>(epmf, cpmf, r[2]);
                 acc_add<KAHAN>(epsp, cpsp, r[3]);
                 acc_add<KAHAN>(ehdb, chdb, r[4]);
@@ -428,9 +507,18 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 }
                 mcs += J.mtcc[row] * sqrtf(s / (float)(re - rb));
             }
-            float *v = vals + (size_t)slot * 8;
+            float *v = vals + (size_t)slot * kVals;
             v[0] = evdw * sc.inv_lj0;
-            v[1] = lc * (eele * sc.inv_el1);
+            // electrostatics lc / el1 (eele - cele) as hi + lo: the atoms'
+            // values are up to 1e6 times their sum, so the product is split
+            // exactly (FMA remainders) and both parts are summed
+            {
+                const float k = lc * sc.inv_el1;
+                const float kr = fmaf(lc, sc.inv_el1, -k);
+                const float hi = k * eele;
+                v[1] = hi;
+                v[8] = fmaf(k, eele, -hi) + fmaf(kr, eele, -k * cele);
+            }
             v[2] = epmf;
             v[3] = epsp;
             v[4] = ehdb;
@@ -441,12 +529,12 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         __syncthreads();
 
         // ---- per-replica sums: thread (g, k) adds term k over the atoms ----
-        for (int t = threadIdx.x; t < R * 8; t += blockDim.x) {
-            const int gg = t >> 3, k = t & 7;
+        for (int t = threadIdx.x; t < R * kVals; t += blockDim.x) {
+            const int gg = t / kVals, k = t - gg * kVals;
             float s = 0.f, c = 0.f;
-            const float *v = vals + (size_t)gg * NL * 8 + k;
-            for (int i = 0; i < NL; ++i) acc_add<KAHAN>(s, c, v[(size_t)i * 8]);
-            rsum[gg * 16 + k] = s;
+            const float *v = vals + (size_t)gg * NL * kVals + k;
+            for (int i = 0; i < NL; ++i) acc_add<KAHAN>(s, c, v[(size_t)i * kVals]);
+            rsum[gg * 16 + k] = s - c;
         }
         __syncthreads();
 
@@ -455,6 +543,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             float e[GD_MAXWEI];
             const float *pm = prop + gg * 8;
             for (int k = 0; k < 6; ++k) e[k] = rsum[gg * 16 + k];
+            e[GD_E_ELE] += rsum[gg * 16 + 8];
             e[GD_E_HDB] = e[GD_E_HDB] * (-1.0f / sqrtf(2.0f * GD_PI));
             e[GD_E_KDE] = rsum[gg * 16 + 6] * sc.kde3inv;
             e[GD_E_LHM] = logf(rsum[gg * 16 + 7] / (float)J.nrow);
