@@ -58,6 +58,14 @@ Shape plain_shape(int natom)
     return s;
 }
 
+int check_stream_range(const JobRes &j, long s1, long s_end)
+{
+    if (j.kstream_steps > 0 && s_end - 1 > j.kstream_steps)
+        return fail(GD_ERR_ARG, "the job's explicit stream words end at step " + std::to_string(j.kstream_steps) +
+                                    ", the launch needs step " + std::to_string(s_end - 1));
+    return GD_OK;
+}
+
 int ensure_records(gd_ctx *ctx, JobRes &j, int cap)
 {
     if (j.rec_cap >= cap && j.rec.p) return GD_OK;
@@ -493,6 +501,8 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     for (int k = 0; k < 6; ++k) h.scale[k] = cx->mcpara.move_scale[k];
     h.key0 = (uint32_t)seed;
     h.key1 = (uint32_t)(seed >> 32);
+    h.kstream = nullptr;
+    h.kstream_steps = 0;
     GD_HIP(hipMemcpyAsync(j.self.p, &h, sizeof(DevJob), hipMemcpyHostToDevice, s));
     j.n_rep = z.n_rep;
     j.n_lig = z.n_lig;
@@ -518,6 +528,36 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     return GD_OK;
 }
 
+int gd_set_stream(gd_ctx *ctx, int job_id, uint64_t seed, const uint32_t *k_words, int n_steps)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j) return fail(GD_ERR_STATE, "unknown job");
+    if (k_words && n_steps <= 0) return fail(GD_ERR_ARG, "explicit stream words need n_steps > 0");
+    hipStream_t s = ctx->stream;
+    DevJob &h = j->hj;
+    h.key0 = (uint32_t)seed;
+    h.key1 = (uint32_t)(seed >> 32);
+    j->seed = seed;
+    if (k_words) {
+        const size_t n = (size_t)j->n_rep * n_steps * 7;
+        for (size_t i = 0; i < n; ++i)
+            if (k_words[i] > 0x7fffffffu) return fail(GD_ERR_ARG, "stream word above RAND_MAX (2^31 - 1)");
+        GD_HIP(upload(j->kstream, k_words, n * sizeof(uint32_t), s));
+        h.kstream = j->kstream.as<uint32_t>();
+        h.kstream_steps = n_steps;
+        j->kstream_steps = n_steps;
+    } else {
+        j->kstream.release();
+        h.kstream = nullptr;
+        h.kstream_steps = 0;
+        j->kstream_steps = 0;
+    }
+    GD_HIP(hipMemcpyAsync(j->self.p, &h, sizeof(DevJob), hipMemcpyHostToDevice, s));
+    GD_HIP(hipStreamSynchronize(s));
+    return GD_OK;
+}
+
 int gd_free_job(gd_ctx *ctx, int job_id)
 {
     if (int rc = check_ctx(ctx)) return rc;
@@ -532,6 +572,7 @@ int gd_montecarlo(gd_ctx *ctx, int job_id, int s1, int s2max)
     JobRes *j = find_job(ctx, job_id);
     if (!j) return fail(GD_ERR_STATE, "unknown job");
     if (s1 < 0 || s2max <= 0) return fail(GD_ERR_ARG, "bad step range");
+    if (int rc = check_stream_range(*j, s1, (long)s1 + s2max)) return rc;
     if (int rc = ensure_records(ctx, *j, std::max(s2max, GD_STEPS_PER_DUMP))) return rc;
     const TargetRes &t = ctx->targets.at(j->target);
     const Shape sh = plain_shape(j->natom);

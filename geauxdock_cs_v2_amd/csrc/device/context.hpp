@@ -52,12 +52,13 @@ struct TargetRes {
 
 struct JobRes {
     int target = -1;
-    DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, psp_pt, psp_row, rep, rec, next_entry, self;
+    DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, psp_pt, psp_row, rep, rec, next_entry, self, kstream;
     DevJob hj{};       // host copy of the descriptor
     int n_rep = 0, n_lig = 0, n_prt = 0, n_tmp = 0, natom = 0, nrow = 0;
     std::vector<int> rep_prt; // protein conformation of every replica (never changes)
     int rec_cap = 0;
     uint64_t seed = 0;
+    int kstream_steps = 0; // steps covered by explicit stream words (0: Philox)
     size_t bytes = 0;
     int steps_done = 0;      // steps advanced by batched runs
     int exchange_rounds = 0; // exchange rounds done by batched runs
@@ -94,6 +95,8 @@ struct gd_ctx {
 namespace gd {
 hipError_t upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s);
 int ensure_records(gd_ctx *ctx, JobRes &j, int cap);
+// fails unless the job's explicit stream words (if any) cover steps s1 .. s_end - 1
+int check_stream_range(const JobRes &j, long s1, long s_end);
 int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
               int waves, int max_rep, bool compensated, bool records);
 } // namespace gd

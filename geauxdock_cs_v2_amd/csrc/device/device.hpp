@@ -102,6 +102,10 @@ struct DevJob {
     float pocket[3]; // the job's pocket centre, target frame
     float scale[6];
     uint32_t key0, key1;
+    // explicit move/accept words k (u = k / RAND_MAX) for steps 1..kstream_steps,
+    // [idx_rep][step - 1][7]; null: the keyed Philox stream
+    const uint32_t *kstream;
+    int kstream_steps;
 };
 
 // A task is the set of replicas one workgroup advances: replicas

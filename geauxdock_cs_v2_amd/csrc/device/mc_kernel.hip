@@ -343,12 +343,18 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         if (MODE == MODE_MC) {
             for (int k = 0; k < 6; ++k) m[k] = st[g * 16 + k];
             if (step != 0) {
-                uint32_t b0 = (uint32_t)step, b1 = idx_rep, b2 = 0u, b3 = 0u;
-                uint32_t c0 = (uint32_t)step, c1 = idx_rep, c2 = 0u, c3 = 1u;
-                philox10(b0, b1, b2, b3, J.key0, J.key1);
-                philox10(c0, c1, c2, c3, J.key0, J.key1);
-                const float u[7] = {uniform31(b0), uniform31(b1), uniform31(b2), uniform31(b3),
-                                    uniform31(c0), uniform31(c1), uniform31(c2)};
+                float u[7];
+                if (J.kstream) { // explicit words, checked by the host to cover the step
+                    const uint32_t *w = J.kstream + ((size_t)idx_rep * J.kstream_steps + (step - 1)) * 7;
+                    for (int k = 0; k < 7; ++k) u[k] = (float)w[k] * 0x1p-31f;
+                } else {
+                    uint32_t b0 = (uint32_t)step, b1 = idx_rep, b2 = 0u, b3 = 0u;
+                    uint32_t c0 = (uint32_t)step, c1 = idx_rep, c2 = 0u, c3 = 1u;
+                    philox10(b0, b1, b2, b3, J.key0, J.key1);
+                    philox10(c0, c1, c2, c3, J.key0, J.key1);
+                    u[0] = uniform31(b0), u[1] = uniform31(b1), u[2] = uniform31(b2), u[3] = uniform31(b3);
+                    u[4] = uniform31(c0), u[5] = uniform31(c1), u[6] = uniform31(c2);
+                }
                 for (int k = 0; k < 6; ++k) m[k] = move_update(J.scale[k], u[k], m[k]);
                 uacc = u[6];
             }

@@ -43,6 +43,18 @@ class Device:
         L.check(self.lib.gd_upload_job(self.h, target, C.byref(cx.view), int(seed), C.byref(j)))
         return j.value
 
+    def set_stream(self, job, seed, k_words=None):
+        """Re-key the job's stream; k_words (uint32 [n_rep, n_steps, 7], values
+        <= RAND_MAX) replaces the Philox MC draws for steps 1..n_steps."""
+        if k_words is None:
+            L.check(self.lib.gd_set_stream(self.h, job, int(seed), None, 0))
+            return
+        w = np.ascontiguousarray(k_words, np.uint32)
+        if w.ndim != 3 or w.shape[2] != 7:
+            raise ValueError("k_words must be [n_rep, n_steps, 7]")
+        L.check(self.lib.gd_set_stream(self.h, job, int(seed), w.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                       w.shape[1]))
+
     def free_job(self, job):
         L.check(self.lib.gd_free_job(self.h, job))
 

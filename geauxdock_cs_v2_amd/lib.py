@@ -67,7 +67,7 @@ def load(path=None):
 
 
 def _declare_device(lib):
-    vp, i32, u64 = C.c_void_p, C.c_int, C.c_uint64
+    vp, i32, u32, u64 = C.c_void_p, C.c_int, C.c_uint32, C.c_uint64
     pp = C.POINTER(C.c_void_p)
     pi = C.POINTER(C.c_int)
     pf = C.POINTER(C.c_float)
@@ -92,6 +92,7 @@ def _declare_device(lib):
     _sig(lib, "gd_server_destroy", None, [vp])
     _sig(lib, "gd_server_submit", i32, [vp, C.POINTER(D.Complex), i32, u64, pi])
     _sig(lib, "gd_server_poll", i32, [vp, i32, C.POINTER(D.RequestResult), C.POINTER(D.ReplicaMC)])
+    _sig(lib, "gd_set_stream", i32, [vp, i32, u64, C.POINTER(u32), i32])
     _sig(lib, "gd_server_cancel", i32, [vp, i32])
     _sig(lib, "gd_server_drain", i32, [vp])
     _sig(lib, "gd_server_clock_ms", C.c_double, [vp])

@@ -112,6 +112,15 @@ int gd_free_target(gd_ctx *ctx, int target_id);
  * a complex, docked against a resident target.  `seed` keys the job's
  * injected random stream. */
 int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t seed, int *job_id);
+/* The job's injected stream (replaces the reference's time-seeded rand_r,
+ * kernel_cpp_montecarlo_soa.cpp:8-18, 233-234, 376): `seed` re-keys the
+ * Philox stream (MC draws and exchange uniforms); with k_words non-null the
+ * MC draws instead come from the caller's words k in [0, RAND_MAX],
+ * u = (float)k / RAND_MAX as rand_r's, laid out [n_rep][n_steps][7] (per
+ * replica idx_rep, per step 1..n_steps: move slots 0-5, then the accept
+ * draw; step 0 draws nothing).  Launches that need steps beyond n_steps
+ * fail with GD_ERR_ARG.  k_words null returns the job to Philox. */
+int gd_set_stream(gd_ctx *ctx, int job_id, uint64_t seed, const uint32_t *k_words, int n_steps);
 int gd_free_job(gd_ctx *ctx, int job_id);
 /* device bytes a job of these sizes takes (admission by HBM footprint) */
 size_t gd_job_footprint(int lig_natom, int n_lig, int kde_npoint, int mcs_entries, int mcs_nrow,

@@ -325,3 +325,36 @@ def test_far_field_energies_large_protein(lib, ctx):
     say("far-field max rel err vs FP64:", np.array2string(err, precision=2))
     assert np.all(np.isfinite(eg))
     assert err.max() <= REL_TOL, err
+
+
+def test_explicit_stream_words(lib, ctx):
+    """gd_set_stream with explicit words: a job fed the Philox words of seed B
+    as a buffer makes exactly the moves and decisions of a job keyed by B,
+    and launches past the buffer's last step are refused."""
+    from geauxdock_cs_v2_amd import synth
+    cx, _ = synth.load_config("c1")
+    n = cx.size["n_rep"]
+    steps = 40
+    seed_a, seed_b = 101, 202
+    reps = cx.replicas()
+    words = np.array([[[lib.gd_stream_k(seed_b, 0, reps[r].idx_rep, s, k) for k in range(7)]
+                       for s in range(1, steps + 1)] for r in range(n)], np.uint32)
+    t = C.c_int()
+    L.check(lib.gd_upload_target(ctx, C.byref(cx.view), C.byref(t)))
+    jb, jx = C.c_int(), C.c_int()
+    L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), seed_b, C.byref(jb)))
+    L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), seed_a, C.byref(jx)))
+    L.check(lib.gd_set_stream(ctx, jx, seed_a, words.ctypes.data_as(C.POINTER(C.c_uint32)), steps))
+    say("case: explicit stream words vs keyed Philox, c1 plain 0..39")
+    L.check(lib.gd_montecarlo(ctx, jb, 0, steps))
+    L.check(lib.gd_montecarlo(ctx, jx, 0, steps))
+    assert bytes(read_reps(lib, ctx, jb, n)) == bytes(read_reps(lib, ctx, jx, n))
+    # step 40 is past the buffer (it covers steps 1..40: step 40 is the last)
+    L.check(lib.gd_montecarlo(ctx, jx, steps, 1))
+    assert lib.gd_montecarlo(ctx, jx, steps + 1, 1) == L.GD_ERR_ARG
+    bad = words.copy()
+    bad[0, 0, 0] = 0x80000000
+    assert lib.gd_set_stream(ctx, jx, seed_a, bad.ctypes.data_as(C.POINTER(C.c_uint32)), steps) == L.GD_ERR_ARG
+    L.check(lib.gd_free_job(ctx, jb))
+    L.check(lib.gd_free_job(ctx, jx))
+    L.check(lib.gd_free_target(ctx, t))
