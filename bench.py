@@ -20,8 +20,9 @@ Beside it, each in its own section of the JSON line:
   screen    batched screening: 1,024 ligands (lognormal 12-96 atoms, 8
             conformations, 8 temperatures; ~393k replicas) against one
             resident 5,000-point protein in 6 conformations, all jobs in one
-            launch per chunk, on a window of steps; versus config-1-style
-            plain launches (job after job) on a sample of the ligands;
+            launch per chunk, on a window of steps; versus plain launches
+            (one job after another, one workgroup per replica) on a sample of
+            the ligands;
   serve     the serving loop on an 8,000-point protein in 4 conformations:
             requests of 10-120 atoms, 4-32 conformations, 8 temperatures,
             500-5,000 steps arriving as a Poisson stream at 50, 200 and 800
@@ -166,7 +167,7 @@ def run_screen(args, dev, perf, synth):
         f"({dev_ms:.1f} ms device), {value:.4g} poses/s, {flop / (dev_ms * 1e-3) / 1e12:.1f} TFLOP/s")
     for j in jobs:
         dev.free_job(j)
-    # config-1-style plain launches, ligand after ligand, on every 64th ligand
+    # plain launches, one ligand after another, on every 64th ligand
     sample = list(range(0, len(cxs), 64))
     plain_poses = 0
     plain_jobs = [dev.upload_job(t, cxs[i], seed=100 + i) for i in sample]

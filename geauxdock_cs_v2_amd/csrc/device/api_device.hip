@@ -196,13 +196,15 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     // conformations (in the float coordinates the kernel reads): within a
     // segment points go nearest first, so the points beyond any radius are a
     // suffix of the segment
-    std::vector<double> rmin(np, 1e300);
+    std::vector<double> rmin(np, 1e300), rmax(np, 0.0);
     for (int c = 0; c < nconf; ++c)
         for (int i = 0; i < np; ++i) {
             const size_t k = (size_t)c * np + i;
             const double x = local(cx->prt_x[k], origin[0]), y = local(cx->prt_y[k], origin[1]),
                          z = local(cx->prt_z[k], origin[2]);
-            rmin[i] = std::min(rmin[i], std::sqrt(x * x + y * y + z * z));
+            const double r = std::sqrt(x * x + y * y + z * z);
+            rmin[i] = std::min(rmin[i], r);
+            rmax[i] = std::max(rmax[i], r);
         }
     std::vector<int> ord(np);
     std::iota(ord.begin(), ord.end(), 0);
@@ -265,7 +267,10 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     // below e^-30 of hdb1.  A point whose smallest radius is r is at least
     // r - rho from every atom within rho of the origin, so per segment the
     // points at r >= rho + far_d are far for all of them; ladder[seg][k] is
-    // the first such point for rho + far_d <= kFarR0 + k * kFarDr.
+    // the first such point for rho + far_d <= kFarR0 + k * kFarDr.  Likewise
+    // a point whose largest radius is r is at least rho' - r from every atom
+    // at radius >= rho' (a ligand that has left the pocket): ladder_in[seg][k]
+    // ends the segment's leading run of points with largest radius <= rung k.
     double far_d = std::max(9.0, 1.0 / (double)ep.el1);
     {
         const double tp_ = 26.0 * std::log(2.0);
@@ -282,13 +287,16 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             }
         far_d += 0.05;
     }
-    std::vector<int> ladder(seg.size() * kFarRungs);
+    std::vector<int> ladder(seg.size() * kFarRungs), ladder_in(seg.size() * kFarRungs);
     for (size_t g = 0; g < seg.size(); ++g)
         for (int k = 0; k < kFarRungs; ++k) {
             const double rk = kFarR0 + k * kFarDr;
             int i = seg[g].y;
             while (i < seg[g].z && rmin[ord[i]] < rk) ++i;
             ladder[g * kFarRungs + k] = i;
+            int m = seg[g].y;
+            while (m < seg[g].z && rmax[ord[m]] <= rk) ++m;
+            ladder_in[g * kFarRungs + k] = m;
         }
 
     const int id = ctx->next_id++;
@@ -298,7 +306,9 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     GD_HIP(upload(t.pp, pp.data(), sizeof(float) * pp.size(), ctx->stream));
     GD_HIP(upload(t.seg, seg.data(), sizeof(int4) * seg.size(), ctx->stream));
     GD_HIP(upload(t.ladder, ladder.data(), sizeof(int) * ladder.size(), ctx->stream));
+    GD_HIP(upload(t.ladder_in, ladder_in.data(), sizeof(int) * ladder_in.size(), ctx->stream));
     t.dt.ladder = t.ladder.as<int>();
+    t.dt.ladder_in = t.ladder_in.as<int>();
     t.dt.far_d = (float)far_d;
     t.dt.pt = t.pt.as<float4>();
     t.dt.pa = t.pa.as<float2>();
@@ -341,7 +351,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         t.seg_host.push_back(g.y);
         t.seg_host.push_back(g.z);
     }
-    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes;
+    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes + t.ladder_in.bytes;
     t.dev_index.assign(np, 0);
     for (int i = 0; i < np; ++i) t.dev_index[ord[i]] = i;
     t.prt_t.assign(cx->prt_t, cx->prt_t + np);

@@ -39,7 +39,7 @@ struct DevBuf {
 };
 
 struct TargetRes {
-    DevBuf pt, pa, pp, seg, ladder;
+    DevBuf pt, pa, pp, seg, ladder, ladder_in;
     DevTarget dt{};
     DevScalars sc{};
     int n_prt = 0, npoint = 0;

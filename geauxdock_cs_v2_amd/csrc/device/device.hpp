@@ -13,6 +13,8 @@
 //                                     smallest radius over the conformations
 //           ladder int[nseg][kFarRungs] first point of each segment at or
 //                                     beyond each radius rung (far field)
+//           ladder_in int[nseg][kFarRungs] end of each segment's leading points
+//                                     whose largest radius is within the rung
 //           pp   float[31][24][PP_N]  type-pair parameters, see PairParam
 //   job     lig3 float4[nlig][natom]  {x3, y3, z3, c3} type-sorted atoms
 //           lig2 float4[nlig][natom]  {x2, y2, z2, 0} file-order atoms (MCS)
@@ -77,6 +79,7 @@ struct DevTarget {
     const float *pp;   // [31][24][PP_N] type-pair parameters
     const int4 *seg;   // [nseg] {point type, begin, end, hydrophobic flag}
     const int *ladder; // [nseg][kFarRungs] first point of the segment at radius >= rung
+    const int *ladder_in; // [nseg][kFarRungs] end of the segment's leading points with largest radius <= rung
     float far_d;       // distance beyond which only LJ and 1/d electrostatics vary
     int npoint, npad, nseg, nconf;
 };

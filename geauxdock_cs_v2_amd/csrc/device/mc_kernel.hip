@@ -251,6 +251,13 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
+
 } // namespace
 
 size_t mc_lds_bytes(int waves, int nrep)
@@ -385,17 +392,27 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // ---- pair terms over this slice's points ----
         float evdw = 0.f, eele = 0.f, epmf = 0.f, epsp = 0.f, ehdb = 0.f, ehpc = 0.f;
         float cvdw = 0.f, cele = 0.f, cpmf = 0.f, cpsp = 0.f, chdb = 0.f, chpc = 0.f;
-        // far field: rho = the wave's largest atom radius; from rung kf of a
-        // segment's ladder on, points are at least far_d from every atom here
-        const float rho = sqrtf(wave_max(fmaf(me.x, me.x, fmaf(me.y, me.y, me.z * me.z))));
-        const int kf = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho + tg.far_d - kFarR0) * (1.0f / kFarDr))));
+        // far field: with the wave's atoms at radii [rho_in, rho_out], from
+        // rung kf of a segment's ladder on points are at least far_d beyond
+        // every atom, and up to ladder_in's rung ki they are at least far_d
+        // inside every atom (a ligand that has left the pocket)
+        const float r2 = fmaf(me.x, me.x, fmaf(me.y, me.y, me.z * me.z));
+        const float rho_out = sqrtf(wave_max(r2)), rho_in = sqrtf(wave_min(r2));
+        const int kf = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho_out + tg.far_d - kFarR0) * (1.0f / kFarDr))));
+        const int ki = __builtin_amdgcn_readfirstlane(
+            min(kFarRungs - 1, (int)floorf((rho_in - tg.far_d - kFarR0) * (1.0f / kFarDr))));
         const cint *ladder = (const cint *)tg.ladder;
+        const cint *ladder_in = (const cint *)tg.ladder_in;
         for (int si = 0; si < tg.nseg; ++si) {
             const int4 sg = tg.seg[si];
             const int b = max(sg.y, lo), e = min(sg.z, hi);
             if (b >= e) continue;
-            const int fb = kf < kFarRungs ? ladder[si * kFarRungs + kf] : sg.z;
-            const int mid = min(max(fb, b), e); // near [b, mid), far [mid, e)
+            const int ie = ki >= 0 ? ladder_in[si * kFarRungs + ki] : sg.y;
+            const int ob = kf < kFarRungs ? ladder[si * kFarRungs + kf] : sg.z;
+            // far [b, m1), near [m1, mid), far [mid, e); the two far runs
+            // cannot share a point, so ie <= ob
+            const int m1 = min(max(ie, b), e);
+            const int mid = min(max(ob, m1), e);
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
@@ -405,7 +422,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             // plain FP32 sums over blocks of at most kBlock points, block sums
             // added with compensation: the electrostatic sum cancels by up to
             // 1e4 over a pose, which long running FP32 sums cannot hold
-            for (int pb = b; pb < mid; pb += kBlock) {
+            for (int pb = m1; pb < mid; pb += kBlock) {
                 const int pe = min(pb + kBlock, mid);
                 Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (sg.w) { // segment flag: hydrophobic-contact points
@@ -426,14 +443,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 acc_add<KAHAN>(ehdb, chdb, bs.h);
                 acc_add<KAHAN>(ehpc, chpc, bs.q);
             }
-            for (int pb = mid; pb < e; pb += kBlock) {
-                const int pe = min(pb + kBlock, e);
-                Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                pair_block_far(ptc, pb, pe, me, q, sc, bs);
-                acc_add<KAHAN>(evdw, cvdw, bs.v);
-                acc_add<KAHAN>(eele, cele, bs.e);
-                acc_add<KAHAN>(eele, cele, -bs.ec);
-                acc_add<KAHAN>(epmf, cpmf, bs.p);
+            for (int part = 0; part < 2; ++part) {
+                const int fb = part ? mid : b, fe = part ? e : m1;
+                for (int pb = fb; pb < fe; pb += kBlock) {
+                    const int pe = min(pb + kBlock, fe);
+                    Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    pair_block_far(ptc, pb, pe, me, q, sc, bs);
+                    acc_add<KAHAN>(evdw, cvdw, bs.v);
+                    acc_add<KAHAN>(eele, cele, bs.e);
+                    acc_add<KAHAN>(eele, cele, -bs.ec);
+                    acc_add<KAHAN>(epmf, cpmf, bs.p);
+                }
             }
         }
         // pocket-specific term: the job's class-2 points whose residue has a
