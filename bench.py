@@ -106,6 +106,7 @@ def run_c2(args, dev, perf, synth):
     t0 = time.perf_counter()
     dev_ms = dev.batch_run([j], args.steps, exchange=exch, graph=True, **shape)
     wall = time.perf_counter() - t0
+    shape_used = dev.job_shape(j)
     value = n_rep * args.steps / wall
     launches = math.ceil(args.steps / chunk)
     launch_ms = dev_ms / launches  # one Monte Carlo launch per chunk (exchange and step advance are ~10 us)
@@ -136,7 +137,9 @@ def run_c2(args, dev, perf, synth):
     roof = {"bound": "valu", "achieved": achieved, "peak": perf.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / perf.FP32_PEAK_TFLOPS, "traffic": traffic,
             "kernel": "mc_kernel", "launch_ms": launch_ms, "steps_per_launch": steps_per_launch,
-            "flop_per_pose": f_pose, "fma_issue_peak": perf.FP32_FMA_TFLOPS}
+            "flop_per_pose": f_pose, "fma_issue_peak": perf.FP32_FMA_TFLOPS,
+            "shape": {"waves": shape_used[0], "slot_waves": shape_used[1], "point_slices": shape_used[2],
+                      "replicas_per_group": shape_used[3], "chosen_by": "given" if args.shape else "autotuner"}}
     if traffic:
         roof["hbm_gbs"] = traffic / (launch_ms * 1e-3) / 1e9
         roof["hbm_frac"] = roof["hbm_gbs"] / perf.HBM_PEAK_GBS

@@ -176,6 +176,20 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
 
 using namespace gd;
 
+extern "C" int gd_job_shape(gd_ctx *ctx, int job_id, int *shape)
+{
+    if (!ctx || !shape) return fail(GD_ERR_ARG, "null argument");
+    auto it = ctx->jobs.find(job_id);
+    if (it == ctx->jobs.end()) return fail(GD_ERR_STATE, "unknown job");
+    const Shape &s = it->second.last_shape;
+    if (s.waves <= 0) return fail(GD_ERR_STATE, "the job has not run in a batched launch");
+    shape[0] = s.waves;
+    shape[1] = s.slot_waves;
+    shape[2] = s.slices;
+    shape[3] = s.nrep;
+    return GD_OK;
+}
+
 extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
                             double *elapsed_ms)
 {
@@ -224,6 +238,7 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
     }
     int max_rep = 1;
     for (const Shape &sh : shapes) max_rep = std::max(max_rep, sh.nrep);
+    for (int i = 0; i < n_jobs; ++i) js[i]->last_shape = shapes[i];
 
     // device job table for this batch
     std::vector<DevJob> table(n_jobs);

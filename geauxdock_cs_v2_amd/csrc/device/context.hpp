@@ -50,6 +50,12 @@ struct TargetRes {
     size_t bytes = 0;
 };
 
+// Workgroup shape of one task: R replicas x N_L atoms in slot_waves waves,
+// times `slices` protein-point slices.
+struct Shape {
+    int nrep, slot_waves, slices, waves;
+};
+
 struct JobRes {
     int target = -1;
     DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, psp_pt, psp_row, rep, rec, next_entry, self, kstream;
@@ -62,12 +68,7 @@ struct JobRes {
     size_t bytes = 0;
     int steps_done = 0;      // steps advanced by batched runs
     int exchange_rounds = 0; // exchange rounds done by batched runs
-};
-
-// Workgroup shape of one task: R replicas x N_L atoms in slot_waves waves,
-// times `slices` protein-point slices.
-struct Shape {
-    int nrep, slot_waves, slices, waves;
+    Shape last_shape{0, 0, 0, 0}; // workgroup shape of the last batched launch
 };
 
 Shape plain_shape(int natom);

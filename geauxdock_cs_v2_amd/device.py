@@ -55,6 +55,13 @@ class Device:
         L.check(self.lib.gd_set_stream(self.h, job, int(seed), w.ctypes.data_as(C.POINTER(C.c_uint32)),
                                        w.shape[1]))
 
+    def job_shape(self, job):
+        """(waves, slot waves, point slices, replicas per workgroup) of the
+        job's last batched launch."""
+        s = (C.c_int * 4)()
+        L.check(self.lib.gd_job_shape(self.h, job, s))
+        return tuple(s)
+
     def free_job(self, job):
         L.check(self.lib.gd_free_job(self.h, job))
 

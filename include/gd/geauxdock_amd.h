@@ -166,6 +166,10 @@ typedef struct gd_batch_opts {
 
 int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
                  double *elapsed_ms);
+/* Workgroup shape of the job's last batched launch (no counterpart in the
+ * reference, whose block shape is a build constant, TperB): shape[0..3] =
+ * waves per workgroup, slot waves, point slices, replicas per workgroup. */
+int gd_job_shape(gd_ctx *ctx, int job_id, int *shape);
 
 /* ------------------------------------------------------------------ */
 /* serving: continuous batching of docking requests on one GPU          */
