@@ -11,8 +11,7 @@ shape, hipGraph-replayed 100-step chunks + exchange), inputs resident in HBM.
 
 Beside it, each in its own section of the JSON line:
   plain     the plain form (one workgroup per replica, one launch per
-            100-step chunk, record read-back per chunk) on a window of the
-            same workload;
+            100-step chunk, record read-back per chunk) over the same steps;
   roofline  the Monte Carlo kernel's algorithmic FLOP rate: FLOP per pose x
             poses per launch / the average launch time of the timed region
             (hipEvents on the launch stream); traffic from the committed
@@ -27,7 +26,9 @@ Beside it, each in its own section of the JSON line:
             requests of 10-120 atoms, 4-32 conformations, 8 temperatures,
             500-5,000 steps arriving as a Poisson stream at 50, 200 and 800
             per second; p50/p99 job latency and achieved pose evaluations/s
-            per rate, versus serial plain launches of the same requests;
+            per rate, versus the same requests run one after another with
+            plain launches (their throughput, and the latencies the same
+            arrivals would see);
   cpu_baseline  the CPU oracle (OpenMP restatement of MonteCarlo_d) on a
             bounded sample of the headline workload.
 """
@@ -205,19 +206,26 @@ def run_serve(args, dev, perf, synth, Server):
     steps = [l["steps"] for l in info["ligands"]]
     poses = [c.size["n_rep"] * s for c, s in zip(cxs, steps)]
     t = dev.upload_target(cxs[0])
-    # serial plain launches of the same requests, first 100 steps of each
-    plain_poses = 0
-    pj = [dev.upload_job(t, c, seed=500 + i) for i, c in enumerate(cxs)]
-    dev.montecarlo(pj[0], 0, 1)
-    t0 = time.perf_counter()
-    for c, j in zip(cxs, pj):
-        dev.montecarlo(j, 1, 100)
-        dev.records(j, c.size["n_rep"], 100)
-        plain_poses += c.size["n_rep"] * 100
-    plain_value = plain_poses / (time.perf_counter() - t0)
-    for j in pj:
+    # serial plain launches of the same requests, every step of each: one
+    # 100-step launch per chunk, record read-back, exchange every 100 steps
+    # (the server's own cadence)
+    warm = dev.upload_job(t, cxs[0], seed=499)
+    dev.montecarlo(warm, 0, 1)
+    dev.free_job(warm)
+    plain_dur = []
+    for i, c in enumerate(cxs):
+        j = dev.upload_job(t, c, seed=500 + i)
+        t0 = time.perf_counter()
+        for k, s1 in enumerate(range(0, steps[i], 100)):
+            s2 = min(100, steps[i] - s1)
+            dev.montecarlo(j, s1, s2)
+            dev.records(j, c.size["n_rep"], 100)
+            if s2 == 100:
+                dev.exchange(j, k)
+        plain_dur.append(time.perf_counter() - t0)
         dev.free_job(j)
-    log(f"serve plain: {len(cxs)} requests x 100 steps one after another, {plain_value:.4g} poses/s")
+    plain_value = sum(poses) / sum(plain_dur)
+    log(f"serve plain: {len(cxs)} requests run one after another, {sum(plain_dur):.2f} s, {plain_value:.4g} poses/s")
     rates = []
     rng = np.random.default_rng(7)
     for lam in synth.SERVE["rates"]:
@@ -238,18 +246,28 @@ def run_serve(args, dev, perf, synth, Server):
         lat = np.array([r.t_done_ms - r.t_submit_ms for r in res])
         span = max(r.t_done_ms for r in res) - min(r.t_submit_ms for r in res)
         achieved = sum(poses) / (span * 1e-3)
+        # the same arrivals served one after another by plain launches
+        # (measured durations, first come first served)
+        free_at, plain_lat = 0.0, []
+        for a, dur in zip(arrivals, plain_dur):
+            start = max(a, free_at)
+            free_at = start + dur
+            plain_lat.append((free_at - a) * 1e3)
         rates.append({"rate_per_s": lam, "requests": len(ids), "p50_ms": float(np.percentile(lat, 50)),
                       "p99_ms": float(np.percentile(lat, 99)), "value": achieved,
-                      "offered": lam * float(np.mean(poses)), "speedup_vs_plain": achieved / plain_value})
+                      "offered": lam * float(np.mean(poses)), "speedup_vs_plain": achieved / plain_value,
+                      "plain_p50_ms": float(np.percentile(plain_lat, 50)),
+                      "plain_p99_ms": float(np.percentile(plain_lat, 99))})
         log(f"serve {lam:g}/s: p50 {rates[-1]['p50_ms']:.0f} ms, p99 {rates[-1]['p99_ms']:.0f} ms, "
-            f"{achieved:.4g} poses/s (offered {rates[-1]['offered']:.4g})")
+            f"{achieved:.4g} poses/s (offered {rates[-1]['offered']:.4g}); serial plain p50 "
+            f"{rates[-1]['plain_p50_ms']:.0f} ms, p99 {rates[-1]['plain_p99_ms']:.0f} ms")
     dev.free_target(t)
     return {"workload": f"serve: {len(cxs)} requests per rate, {min(l['n_atoms'] for l in info['ligands'])}-"
                         f"{max(l['n_atoms'] for l in info['ligands'])} atoms, 4-32 conf, 8 temperatures, "
                         f"{min(steps)}-{max(steps)} steps, against an {cxs[0].size['prt_npoint']}-point protein "
                         f"x {cxs[0].size['n_prt']} conf",
-            "rates": rates, "plain": {"value": plain_value, "sample": "each request's first 100 steps, "
-                                                                       "serially, plain launches"}}
+            "rates": rates, "plain": {"value": plain_value, "sample": "every step of each request, one request "
+                                                                       "after another, plain launches"}}
 
 
 def main():
@@ -257,7 +275,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--plain-steps", type=int, default=-1,
+                    help="steps of the plain-form comparison run (default: the same steps as the batched run)")
     ap.add_argument("--sections", default="c2,screen,serve",
                     help="comma list of c2, screen, serve (c2 gives value and roofline)")
     ap.add_argument("--shape", default="", help="W,S: fixed workgroup shape for c2 (skips the autotuner)")
@@ -287,6 +306,8 @@ def main():
     sections = set(s.strip() for s in args.sections.split(",") if s.strip())
     # without c2 (profiling runs of one section) the line has no headline value
 
+    if args.plain_steps < 0:
+        args.plain_steps = args.steps
     from geauxdock_cs_v2_amd import perf, synth
     from geauxdock_cs_v2_amd.device import Device, Server
 
