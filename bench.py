@@ -29,6 +29,9 @@ Beside it, each in its own section of the JSON line:
             per rate, versus the same requests run one after another with
             plain launches (their throughput, and the latencies the same
             arrivals would see);
+  stress    the stress shape at full size (65,536 replicas, 150 atoms,
+            20,000-point protein in 32 conformations) on a 20-step window,
+            batched and plain;
   cpu_baseline  the CPU oracle (OpenMP restatement of MonteCarlo_d) on a
             bounded sample of the headline workload.
 """
@@ -270,6 +273,43 @@ def run_serve(args, dev, perf, synth, Server):
                                                                        "after another, plain launches"}}
 
 
+def run_stress(args, dev, perf, synth):
+    """The stress shape at full size: 65,536 replicas of a 150-atom ligand
+    against a 20,000-point protein in 32 conformations, on a window of steps;
+    batched against plain launches over the same steps."""
+    cx, info = synth.load_config("c5")
+    n_rep = cx.size["n_rep"]
+    f_pose = perf.flops_per_pose(cx)
+    steps = args.stress_steps
+    t = dev.upload_target(cx)
+    j = dev.upload_job(t, cx, seed=5)
+    dev.batch_run([j], 1, graph=False)  # warm-up: the autotuner's trials (one step each at this size)
+    t0 = time.perf_counter()
+    dev_ms = dev.batch_run([j], steps, exchange=info["cfg"]["exchange"], graph=True)
+    wall = time.perf_counter() - t0
+    value = n_rep * steps / wall
+    shape = dev.job_shape(j)
+    dev.free_job(j)
+    jp = dev.upload_job(t, cx, seed=5)
+    dev.montecarlo(jp, 0, 1)
+    t0 = time.perf_counter()
+    dev.montecarlo(jp, 1, steps)
+    dev.records(jp, n_rep, max(steps, 100))
+    plain_wall = time.perf_counter() - t0
+    dev.free_job(jp)
+    dev.free_target(t)
+    plain_value = n_rep * steps / plain_wall
+    tflops = f_pose * n_rep * steps / (dev_ms * 1e-3) / 1e12
+    log(f"stress batched: {steps} steps x {n_rep} replicas in {wall:.3f} s, {value:.4g} poses/s, {tflops:.1f} TFLOP/s; "
+        f"plain {plain_value:.4g} poses/s")
+    return {"workload": f"stress: ligand {cx.size['lig_natom']} atoms x {cx.size['n_lig']} conf, protein "
+                        f"{cx.size['prt_npoint']} points x {cx.size['n_prt']} conf, {cx.size['n_tmp']} temperatures, "
+                        f"{n_rep} replicas", "steps": steps, "value": value, "tflops": tflops,
+            "shape": {"waves": shape[0], "slot_waves": shape[1], "point_slices": shape[2],
+                      "replicas_per_group": shape[3]},
+            "plain": {"value": plain_value, "steps": steps}, "speedup_vs_plain": value / plain_value}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,13 +317,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=-1,
                     help="steps of the plain-form comparison run (default: the same steps as the batched run)")
-    ap.add_argument("--sections", default="c2,screen,serve",
-                    help="comma list of c2, screen, serve (c2 gives value and roofline)")
+    ap.add_argument("--sections", default="c2,screen,serve,stress",
+                    help="comma list of c2, screen, serve, stress (c2 gives value and roofline)")
     ap.add_argument("--shape", default="", help="W,S: fixed workgroup shape for c2 (skips the autotuner)")
     ap.add_argument("--screen-steps", type=int, default=100)
     ap.add_argument("--screen-plain-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-hbm-gib", type=int, default=64)
+    ap.add_argument("--stress-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -315,6 +356,7 @@ def main():
     cx, c2 = run_c2(args, dev, perf, synth) if "c2" in sections else (None, None)
     screen = run_screen(args, dev, perf, synth) if "screen" in sections else None
     serve = run_serve(args, dev, perf, synth, Server) if "serve" in sections else None
+    stress = run_stress(args, dev, perf, synth) if "stress" in sections else None
     dev.close()
 
     cpu = None if (args.no_cpu or cx is None) else cpu_baseline(cx, args.cpu_seconds)
@@ -336,6 +378,7 @@ def main():
         "roofline": c2["roofline"],
         "screen": screen,
         "serve": serve,
+        "stress": stress,
         "cpu_baseline": cpu,
     }
     if serve:
