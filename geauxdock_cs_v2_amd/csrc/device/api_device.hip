@@ -377,7 +377,8 @@ size_t gd_job_footprint(int lig_natom, int n_lig, int kde_npoint, int mcs_entrie
     return 2 * sizeof(float4) * a * n_lig + (sizeof(int) + sizeof(int2)) * a + sizeof(float4) * kde_npoint +
            sizeof(int) * (mcs_nrow + 1) + sizeof(float4) * mcs_entries + sizeof(float) * mcs_nrow +
            sizeof(float) * n_tmp + sizeof(gd_ReplicaMC) * (size_t)n_rep * (1 + record_cap) +
-           sizeof(int) * (size_t)n_rep + sizeof(DevJob) + 11 * 256 // allocation granularity slack
+           3 * sizeof(int) * (size_t)n_rep /* next_entry, McLog counters */ + sizeof(DevJob) +
+           12 * 256 // allocation granularity slack
            + 64 * 1024; // PSP point rows (100 bytes per point, up to ~650 points)
 }
 
@@ -513,6 +514,10 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     h.key1 = (uint32_t)(seed >> 32);
     h.kstream = nullptr;
     h.kstream_steps = 0;
+    GD_HIP(j.mclog.alloc(sizeof(int) * 2 * (size_t)z.n_rep));
+    GD_HIP(hipMemsetAsync(j.mclog.p, 0, j.mclog.bytes, s));
+    h.mc_acc = j.mclog.as<int>();
+    h.ex_acc = h.mc_acc + z.n_rep;
     GD_HIP(hipMemcpyAsync(j.self.p, &h, sizeof(DevJob), hipMemcpyHostToDevice, s));
     j.n_rep = z.n_rep;
     j.n_lig = z.n_lig;
@@ -532,7 +537,7 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     }
     j.bytes = j.lig3.bytes + j.lig2.bytes + j.lt.bytes + j.krange.bytes + j.kde.bytes + j.mrow.bytes + j.ment.bytes +
               j.mtcc.bytes + j.mbeta.bytes + j.psp_pt.bytes + j.psp_row.bytes + j.rep.bytes + j.next_entry.bytes +
-              j.self.bytes;
+              j.self.bytes + j.mclog.bytes;
     GD_HIP(hipStreamSynchronize(s));
     *job_id = id;
     return GD_OK;
@@ -565,6 +570,20 @@ int gd_set_stream(gd_ctx *ctx, int job_id, uint64_t seed, const uint32_t *k_word
     }
     GD_HIP(hipMemcpyAsync(j->self.p, &h, sizeof(DevJob), hipMemcpyHostToDevice, s));
     GD_HIP(hipStreamSynchronize(s));
+    return GD_OK;
+}
+
+int gd_read_mclog(gd_ctx *ctx, int job_id, int *mc_accepts, int *exchange_accepts)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    JobRes *j = find_job(ctx, job_id);
+    if (!j) return fail(GD_ERR_STATE, "unknown job");
+    const size_t n = (size_t)j->n_rep;
+    if (mc_accepts) GD_HIP(hipMemcpyAsync(mc_accepts, j->mclog.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (exchange_accepts)
+        GD_HIP(hipMemcpyAsync(exchange_accepts, j->mclog.as<int>() + n, sizeof(int) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
     return GD_OK;
 }
 

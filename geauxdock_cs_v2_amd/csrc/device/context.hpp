@@ -58,7 +58,8 @@ struct Shape {
 
 struct JobRes {
     int target = -1;
-    DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, psp_pt, psp_row, rep, rec, next_entry, self, kstream;
+    DevBuf lig3, lig2, lt, krange, kde, mrow, ment, mtcc, mbeta, psp_pt, psp_row, rep, rec, next_entry, self, kstream,
+        mclog;
     DevJob hj{};       // host copy of the descriptor
     int n_rep = 0, n_lig = 0, n_prt = 0, n_tmp = 0, natom = 0, nrow = 0;
     std::vector<int> rep_prt; // protein conformation of every replica (never changes)

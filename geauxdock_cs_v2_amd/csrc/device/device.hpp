@@ -109,6 +109,10 @@ struct DevJob {
     // [idx_rep][step - 1][7]; null: the keyed Philox stream
     const uint32_t *kstream;
     int kstream_steps;
+    // McLog counters per replica since upload: accepted MC moves (steps >= 1)
+    // and accepted temperature swaps
+    int *mc_acc;
+    int *ex_acc;
 };
 
 // A task is the set of replicas one workgroup advances: replicas

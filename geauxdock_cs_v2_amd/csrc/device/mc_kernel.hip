@@ -78,6 +78,7 @@ struct Slot {
 
 constexpr int kBlock = 64; // points per plain FP32 partial sum
 constexpr int kVals = 9;   // per-slot terms: vdw, ele hi, pmf, psp, hdb, hpc, kde, mcs, ele lo
+constexpr int kIst = 5;    // ints of replica state per replica in LDS
 
 } // namespace
 
@@ -266,8 +267,8 @@ size_t mc_lds_bytes(int waves, int nrep)
     // one region for the cross-slice partials (7 floats per slot and slice
     // beyond the first) and, after a barrier, the per-slot terms (kVals
     // floats per slot); then rsum, st, prop (16 + 16 + 8 floats per replica)
-    // and 4 ints per replica
-    return sizeof(float) * ((size_t)lanes * kVals + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * 4;
+    // and kIst ints per replica
+    return sizeof(float) * ((size_t)lanes * kVals + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * kIst;
 }
 
 // At least 6 waves per SIMD (at most 80 VGPRs): two 12-wave workgroups share
@@ -306,7 +307,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     float *rsum = lds + (size_t)lanes * kVals; // [R][16] compensated term sums
     float *st = rsum + (size_t)R * 16;     // [R][16] movematrix[6], energy[10]
     float *prop = st + (size_t)R * 16;     // [R][8] proposed move, accept uniform
-    int *ist = (int *)(prop + (size_t)R * 8); // [R][4] next_entry, is_accept, step, idx_tmp
+    int *ist = (int *)(prop + (size_t)R * 8); // [R][kIst] next_entry, is_accept, step, idx_tmp, accepted moves
 
     const int s1 = task.s1 + (step_base ? *step_base : 0);
     const int s2max = task.s2max;
@@ -316,10 +317,11 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             const gd_ReplicaMC &rp = J.rep[task.rep0 + i];
             for (int k = 0; k < 6; ++k) st[i * 16 + k] = rp.movematrix[k];
             for (int k = 0; k < GD_MAXWEI; ++k) st[i * 16 + 6 + k] = rp.energy[k];
-            ist[i * 4 + 0] = 0;
-            ist[i * 4 + 1] = rp.is_accept;
-            ist[i * 4 + 2] = rp.step;
-            ist[i * 4 + 3] = rp.idx_tmp;
+            ist[i * kIst + 0] = 0;
+            ist[i * kIst + 1] = rp.is_accept;
+            ist[i * kIst + 2] = rp.step;
+            ist[i * kIst + 3] = rp.idx_tmp;
+            ist[i * kIst + 4] = 0;
         }
     }
     __syncthreads();
@@ -597,29 +599,30 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 int acc;
                 if (stp != 0) {
                     const float delta = etotal - cur[6 + GD_E_TOTAL];
-                    acc = pm[6] < expf(delta * J.mbeta[ist[gg * 4 + 3]]);
+                    acc = pm[6] < expf(delta * J.mbeta[ist[gg * kIst + 3]]);
                 } else {
                     acc = 1;
                 }
-                ist[gg * 4 + 1] = acc;
+                ist[gg * kIst + 1] = acc;
+                if (acc && stp != 0) ist[gg * kIst + 4] += 1;
                 if (acc) {
                     for (int k = 0; k < 6; ++k) cur[k] = pm[k];
                     for (int k = 0; k < GD_MAXWEI; ++k) cur[6 + k] = e[k];
-                    ist[gg * 4 + 2] = stp;
+                    ist[gg * kIst + 2] = stp;
                     if (RECORDS) {
                         const int r = task.rep0 + gg;
-                        const int ne = ist[gg * 4 + 0];
+                        const int ne = ist[gg * kIst + 0];
                         gd_ReplicaMC *rc = J.rec + (size_t)r * J.rec_cap + ne;
                         const gd_ReplicaMC &src = J.rep[r];
                         rc->idx_rep = src.idx_rep;
                         rc->idx_lig = src.idx_lig;
                         rc->idx_prt = src.idx_prt;
-                        rc->idx_tmp = ist[gg * 4 + 3];
+                        rc->idx_tmp = ist[gg * kIst + 3];
                         for (int k = 0; k < 6; ++k) rc->movematrix[k] = cur[k];
                         for (int k = 0; k < GD_MAXWEI; ++k) rc->energy[k] = cur[6 + k];
                         rc->step = stp;
                         rc->is_accept = src.is_accept;
-                        ist[gg * 4 + 0] = ne + 1;
+                        ist[gg * kIst + 0] = ne + 1;
                     }
                 }
             }
@@ -632,9 +635,10 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             gd_ReplicaMC &rp = J.rep[task.rep0 + i];
             for (int k = 0; k < 6; ++k) rp.movematrix[k] = st[i * 16 + k];
             for (int k = 0; k < GD_MAXWEI; ++k) rp.energy[k] = st[i * 16 + 6 + k];
-            rp.step = ist[i * 4 + 2];
-            rp.is_accept = ist[i * 4 + 1];
-            if (RECORDS) J.next_entry[task.rep0 + i] = ist[i * 4 + 0];
+            rp.step = ist[i * kIst + 2];
+            rp.is_accept = ist[i * kIst + 1];
+            if (RECORDS) J.next_entry[task.rep0 + i] = ist[i * kIst + 0];
+            J.mc_acc[task.rep0 + i] += ist[i * kIst + 4]; // this workgroup owns the replica
         }
     }
 }
@@ -668,6 +672,8 @@ __global__ void exchange_kernel(const DevJob *__restrict__ job, int n_ladder, in
         if (u < exp(delta)) {
             J.rep[ra].idx_tmp = t + 1;
             J.rep[rb].idx_tmp = t;
+            ++J.ex_acc[ra]; // the ladder's replicas belong to this thread alone
+            ++J.ex_acc[rb];
             ++accepted;
         }
     }

@@ -146,6 +146,10 @@ int gd_exchange(gd_ctx *ctx, int job_id, int round, int *n_accept);
 int gd_energy(gd_ctx *ctx, int job_id, int n, const int *rep_idx, const float *move, float *e_out);
 
 int gd_read_replicas(gd_ctx *ctx, int job_id, gd_ReplicaMC *out);
+/* McLog counters (src/common/geauxdock.h:359-368, declared but never filled
+ * in the reference): per replica since upload, accepted MC moves (steps >= 1,
+ * every form) and accepted temperature swaps; either array may be null. */
+int gd_read_mclog(gd_ctx *ctx, int job_id, int *mc_accepts, int *exchange_accepts);
 int gd_write_replicas(gd_ctx *ctx, int job_id, const gd_ReplicaMC *in);
 /* records of the last chunk: out[r * cap + k] for k < next_entry[r] */
 int gd_read_records(gd_ctx *ctx, int job_id, gd_ReplicaMC *out, int *next_entry, int cap);

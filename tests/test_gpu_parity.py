@@ -384,3 +384,32 @@ def test_far_field_detached_ligand(lib, ctx, radius):
     say(f"detached {radius} A max rel err vs FP64:", np.array2string(err, precision=2, max_line_width=200))
     assert np.all(np.isfinite(eg))
     assert err.max() <= REL_TOL, err
+
+
+def test_mclog_counters(lib, ctx, kat_complex11):
+    """McLog counters: accepted MC moves per replica equal the records the
+    plain chunks kept (minus the step-0 entry); accepted swaps per replica
+    equal the rounds in which its temperature label changed."""
+    cx = kat_complex11[1]
+    n = cx.size["n_rep"]
+    t, j = upload(lib, ctx, cx, 41)
+    moves = np.zeros(n, np.int64)
+    swaps = np.zeros(n, np.int64)
+    say("case: McLog counters, kat11, 3 chunks of 50 steps with exchange")
+    for c in range(3):
+        L.check(lib.gd_montecarlo(ctx, j, 50 * c, 50))
+        rec = (D.ReplicaMC * (n * 100))()
+        ne = np.zeros(n, np.int32)
+        L.check(lib.gd_read_records(ctx, j, rec, ne.ctypes.data_as(C.POINTER(C.c_int)), 100))
+        moves += ne - (1 if c == 0 else 0)
+        before = np.array([r.idx_tmp for r in read_reps(lib, ctx, j, n)])
+        L.check(lib.gd_exchange(ctx, j, c, None))
+        after = np.array([r.idx_tmp for r in read_reps(lib, ctx, j, n)])
+        swaps += before != after
+    mc = np.zeros(n, np.int32)
+    ex = np.zeros(n, np.int32)
+    L.check(lib.gd_read_mclog(ctx, j, mc.ctypes.data_as(C.POINTER(C.c_int)), ex.ctypes.data_as(C.POINTER(C.c_int))))
+    free(lib, ctx, t, j)
+    assert np.array_equal(mc, moves)
+    assert np.array_equal(ex, swaps)
+    assert swaps.sum() > 0
