@@ -18,6 +18,7 @@ INC = os.path.join(ROOT, "include")
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libgeauxdock_amd.so")
+APP = os.path.join(PKG, "gd_dock")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -56,6 +57,13 @@ def build(verbose=False, jobs=8):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)
+    # the docking driver (the reference's 3_geauxdock over the C ABI), next
+    # to the library it loads through $ORIGIN
+    tmp = APP + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", os.path.join(CSRC, "app", "gd_dock.cpp"),
+          f"-L{os.path.dirname(LIB)}", "-lgeauxdock_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib",
+          "-o", tmp])
+    os.replace(tmp, APP)
     return LIB
 
 

@@ -413,3 +413,34 @@ def test_mclog_counters(lib, ctx, kat_complex11):
     assert np.array_equal(mc, moves)
     assert np.array_equal(ex, swaps)
     assert swaps.sum() > 0
+
+
+def test_docking_driver_known_answer(lib, tmp_path):
+    """The native driver (the reference's 3_geauxdock over gd::Docker) on a CSV
+    job list holding the 1a07C job: its first PrintCsv line is the
+    known-answer energies (util_print.cpp:351) and the record dump has one
+    block per replica."""
+    import os
+    import subprocess
+    from conftest import ASTEX
+    from geauxdock_cs_v2_amd.lib import PARAM_DIR, PKG_DIR
+    exe = os.path.join(PKG_DIR, "gd_dock")
+    assert os.path.exists(exe), "gd_dock is built by __graft_entry__.build()"
+    cols = ["magicnumber_a4a6b25c", "0", "1a07C", os.path.join(ASTEX, "1a07C-1.pdb"), "1a07C1",
+            os.path.join(ASTEX, "1a07C1.sdf"), os.path.join(ASTEX, "1a07C1-0.8.ff"),
+            os.path.join(PARAM_DIR, "08ff_opt"), os.path.join(PARAM_DIR, "paras"),
+            os.path.join(PARAM_DIR, "08_nor_a"), os.path.join(PARAM_DIR, "08_nor_b"),
+            "1", "0.044", "0.036", "0.02", "0.08", "1"] + ["-2"] * 40
+    jobs = tmp_path / "jobs.csv"
+    jobs.write_text("header\n" + ",".join(cols) + "\n")
+    say("case: gd_dock on the 1a07C job list")
+    r = subprocess.run([exe, str(jobs), "--out", str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.splitlines()
+    assert out[0] == "rep step vdw ele pmf psp hdb hpc kde lhm dst total"
+    f = out[1].split()
+    assert f[0] == "0" and f[1] == "0"
+    assert np.all(np.abs(np.array([float(x) for x in f[2:]]) - KAT_1A07C1) <= 1.5e-4), out[1]
+    dump = (tmp_path / "out_1a07C1.txt").read_text().splitlines()
+    assert sum(l.startswith("replica, ") for l in dump) == 51
