@@ -147,6 +147,8 @@ void gd_destroy(gd_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t g : ctx->group_stream)
+        if (g) (void)hipStreamSynchronize(g);
     ctx->jobs.clear();
     ctx->targets.clear();
     ctx->task_buf.release();
@@ -155,6 +157,8 @@ void gd_destroy(gd_ctx *ctx)
     ctx->pose_move.release();
     ctx->pose_out.release();
     ctx->counter.release();
+    for (hipStream_t &g : ctx->group_stream)
+        if (g) (void)hipStreamDestroy(g);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -621,7 +625,7 @@ int gd_exchange(gd_ctx *ctx, int job_id, int round, int *n_accept)
     if (round < 0) return fail(GD_ERR_ARG, "negative exchange round");
     if (!ctx->counter.p) GD_HIP(ctx->counter.alloc(sizeof(int)));
     GD_HIP(hipMemsetAsync(ctx->counter.p, 0, sizeof(int), ctx->stream));
-    GD_HIP(launch_exchange(j->self.as<DevJob>(), j->n_lig * j->n_prt, j->n_lig, j->n_tmp, round,
+    GD_HIP(launch_exchange(j->self.as<DevJob>(), 0, j->n_lig * j->n_prt, j->n_lig, j->n_tmp, round,
                            ctx->counter.as<int>(), ctx->stream, nullptr));
     int cnt = 0;
     GD_HIP(hipMemcpyAsync(&cnt, ctx->counter.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

@@ -83,27 +83,161 @@ Shape heuristic_shape(int natom, int W, int max_rep_block, int n_prt, int slots)
     return best;
 }
 
-void build_tasks(const JobRes &j, int job_slot, const Shape &sh, int s1, int s2max, std::vector<DevTask> &out)
+// tasks of protein conformation p of a job: its consecutive replicas, R per task
+static void build_unit_tasks(const JobRes &j, int job_slot, const Shape &sh, int p, int s1, int s2max,
+                             std::vector<DevTask> &out)
 {
     const int block = j.n_tmp * j.n_lig; // replicas of one protein conformation are contiguous
-    for (int p = 0; p < j.n_prt; ++p)
-        for (int r = 0; r < block; r += sh.nrep) {
-            DevTask t;
-            t.job = job_slot;
-            t.rep0 = p * block + r;
-            t.nrep = std::min(sh.nrep, block - r);
-            t.prt = p;
-            t.s1 = s1;
-            t.s2max = s2max;
-            t.slices = sh.slices;
-            t.slot_waves = sh.slot_waves;
-            out.push_back(t);
-        }
+    for (int r = 0; r < block; r += sh.nrep) {
+        DevTask t;
+        t.job = job_slot;
+        t.rep0 = p * block + r;
+        t.nrep = std::min(sh.nrep, block - r);
+        t.prt = p;
+        t.s1 = s1;
+        t.s2max = s2max;
+        t.slices = sh.slices;
+        t.slot_waves = sh.slot_waves;
+        out.push_back(t);
+    }
 }
 
+void build_tasks(const JobRes &j, int job_slot, const Shape &sh, int s1, int s2max, std::vector<DevTask> &out)
+{
+    for (int p = 0; p < j.n_prt; ++p) build_unit_tasks(j, job_slot, sh, p, s1, s2max, out);
+}
+
+namespace {
+
+// Workgroup groups.  A launch ends when its slowest workgroup does, so when
+// its task count is not a multiple of the workgroups resident at once the
+// last round leaves CUs idle until the next launch (C2: 880 tasks over 512
+// resident workgroups idles 14% of the GPU).  The batch is therefore split
+// into groups of whole (job, protein conformation) units, each launched on
+// a stream of its own with its own task table, step counters and graph:
+// one group's tail overlaps the other's next launch.  Every temperature
+// ladder lies within one unit, so the exchange rounds split with the
+// groups, and the results do not depend on the split.
+struct Group {
+    struct Range {
+        int job, p0, p1; // protein conformations p0 .. p1 - 1 of batch job `job`
+    };
+    std::vector<Range> ranges;
+    std::vector<DevTask> tasks, tail;
+    DevBuf tasks_dev, bases; // bases: step and exchange-round counters advanced in the graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Group() = default;
+    Group(const Group &) = delete;
+    Group(Group &&o) noexcept
+        : ranges(std::move(o.ranges)), tasks(std::move(o.tasks)), tail(std::move(o.tail)),
+          tasks_dev(std::move(o.tasks_dev)), bases(std::move(o.bases)), graph(o.graph), exec(o.exec)
+    {
+        o.graph = nullptr;
+        o.exec = nullptr;
+    }
+    ~Group()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    int *step_base() const { return bases.as<int>(); }
+    int *round_base() const { return bases.as<int>() + 1; }
+};
+
+// waits for the group streams when it goes out of scope: declared after the
+// groups, it runs first, so no graph or buffer is released while in use
+struct GroupsIdle {
+    gd_ctx *ctx;
+    ~GroupsIdle()
+    {
+        for (hipStream_t g : ctx->group_stream)
+            if (g) (void)hipStreamSynchronize(g);
+    }
+};
+
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() = default;
+    Event(const Event &) = delete;
+    ~Event()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// The units (job i, conformation p) in order, cut into at most G runs of
+// about equal task counts (a unit goes to the run holding its midpoint).
+std::vector<Group> make_groups(const std::vector<JobRes *> &js, const std::vector<Shape> &shapes, int G)
+{
+    long total = 0;
+    for (size_t i = 0; i < js.size(); ++i)
+        total += (long)js[i]->n_prt * ((js[i]->n_tmp * js[i]->n_lig + shapes[i].nrep - 1) / shapes[i].nrep);
+    std::vector<Group> gr(std::max(1, G));
+    long before = 0;
+    for (size_t i = 0; i < js.size(); ++i) {
+        const long c = (js[i]->n_tmp * js[i]->n_lig + shapes[i].nrep - 1) / shapes[i].nrep;
+        for (int p = 0; p < js[i]->n_prt; ++p) {
+            const int g = std::min((int)gr.size() - 1, (int)(((double)before + 0.5 * c) * gr.size() / total));
+            auto &rs = gr[g].ranges;
+            if (!rs.empty() && rs.back().job == (int)i && rs.back().p1 == p)
+                rs.back().p1 = p + 1;
+            else
+                rs.push_back({(int)i, p, p + 1});
+            before += c;
+        }
+    }
+    std::vector<Group> out;
+    for (Group &g : gr)
+        if (!g.ranges.empty()) out.push_back(std::move(g));
+    return out;
+}
+
+// a group's tasks for s2max steps from the batch jobs' own step counts,
+// in decreasing cost order so heavy workgroups start first
+void group_tasks(const Group &g, const std::vector<JobRes *> &js, const std::vector<Shape> &shapes,
+                 const std::vector<int> &s1_of, int s2max, std::vector<DevTask> &out)
+{
+    out.clear();
+    for (const Group::Range &r : g.ranges)
+        for (int p = r.p0; p < r.p1; ++p)
+            build_unit_tasks(*js[r.job], r.job, shapes[r.job], p, s1_of[r.job], s2max, out);
+    std::stable_sort(out.begin(), out.end(), [&](const DevTask &x, const DevTask &y) {
+        return (long)x.nrep * js[x.job]->natom > (long)y.nrep * js[y.job]->natom;
+    });
+}
+
+int ensure_group_streams(gd_ctx *ctx)
+{
+    for (hipStream_t &g : ctx->group_stream)
+        if (!g) GD_HIP(hipStreamCreateWithFlags(&g, hipStreamNonBlocking));
+    return GD_OK;
+}
+
+// order the group streams after the work on s (fork), or s after the group
+// streams (join)
+int fork_groups(gd_ctx *ctx, hipStream_t s, size_t n, hipEvent_t ev)
+{
+    GD_HIP(hipEventRecord(ev, s));
+    for (size_t g = 0; g < n; ++g) GD_HIP(hipStreamWaitEvent(ctx->group_stream[g], ev, 0));
+    return GD_OK;
+}
+
+int join_groups(gd_ctx *ctx, hipStream_t s, size_t n, Event *ev)
+{
+    for (size_t g = 0; g < n; ++g) {
+        GD_HIP(hipEventRecord(ev[g].e, ctx->group_stream[g]));
+        GD_HIP(hipStreamWaitEvent(s, ev[g].e, 0));
+    }
+    return GD_OK;
+}
+
+} // namespace
+
 // Autotuner: time `trial` steps of every candidate workgroup shape on the
-// job's own data (state saved and restored around the trials) and keep the
-// fastest; results are cached per (N_L, N_P, n_rep, n_prt).
+// job's own data (state saved and restored around the trials), launched in
+// workgroup groups as the batched runs are, and keep the fastest; results
+// are cached per (N_L, N_P, n_rep, n_prt).
 int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool compensated, Shape &out)
 {
     const auto key = std::make_tuple(j.natom, t.npoint, j.n_rep, j.n_prt);
@@ -130,12 +264,14 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
             if (!dup) cands.push_back(s);
         }
     if (cands.empty()) return fail(GD_ERR_ARG, "ligand too large for a 16-wave workgroup");
+    if (int rc = ensure_group_streams(ctx)) return rc;
+    hipStream_t s = ctx->stream;
     DevBuf backup;
     GD_HIP(backup.alloc(j.rep.bytes));
-    GD_HIP(hipMemcpyAsync(backup.p, j.rep.p, j.rep.bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    hipEvent_t e0, e1;
-    GD_HIP(hipEventCreate(&e0));
-    GD_HIP(hipEventCreate(&e1));
+    GD_HIP(hipMemcpyAsync(backup.p, j.rep.p, j.rep.bytes, hipMemcpyDeviceToDevice, s));
+    Event e0, e1, ef, ej[gd_ctx::kGroups];
+    for (Event *e : {&e0, &e1, &ef}) GD_HIP(hipEventCreate(&e->e));
+    for (Event &e : ej) GD_HIP(hipEventCreateWithFlags(&e.e, hipEventDisableTiming));
     // two timed steps per candidate; one, untimed warm-up dropped, once a
     // step holds more than ~1e11 pair evaluations (a few 100 ms)
     const double pairs = (double)j.n_rep * j.natom * t.npoint;
@@ -143,27 +279,36 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
     const int trial = large ? 1 : 2;
     float best_ms = 1e30f;
     Shape best = cands[0];
-    for (const Shape &s : cands) {
-        std::vector<DevTask> tasks;
-        build_tasks(j, 0, s, s1, trial, tasks);
+    std::vector<JobRes *> one{&j};
+    const std::vector<int> s1_of{s1};
+    for (const Shape &c : cands) {
+        const std::vector<Shape> shapes{c};
+        std::vector<Group> gr = make_groups(one, shapes, gd_ctx::kGroups);
+        GroupsIdle idle{ctx};
+        for (Group &g : gr) {
+            group_tasks(g, one, shapes, s1_of, trial, g.tasks);
+            GD_HIP(upload(g.tasks_dev, g.tasks.data(), sizeof(DevTask) * g.tasks.size(), s));
+        }
         // warm the code object and caches once, then time
         for (int rep = large ? 1 : 0; rep < 2; ++rep) {
-            GD_HIP(hipEventRecord(e0, ctx->stream));
-            if (int rc = run_tasks(ctx, t, j.self.as<DevJob>(), tasks, s.waves, s.nrep, compensated, false)) return rc;
-            GD_HIP(hipEventRecord(e1, ctx->stream));
-            GD_HIP(hipMemcpyAsync(j.rep.p, backup.p, j.rep.bytes, hipMemcpyDeviceToDevice, ctx->stream));
-            GD_HIP(hipEventSynchronize(e1));
+            GD_HIP(hipEventRecord(e0.e, s));
+            if (int rc = fork_groups(ctx, s, gr.size(), ef.e)) return rc;
+            for (size_t g = 0; g < gr.size(); ++g)
+                GD_HIP(launch_mc(t.dt, t.sc, j.self.as<DevJob>(), gr[g].tasks_dev.as<DevTask>(), (int)gr[g].tasks.size(),
+                                 c.waves, c.nrep, compensated, false, ctx->group_stream[g], nullptr));
+            if (int rc = join_groups(ctx, s, gr.size(), ej)) return rc;
+            GD_HIP(hipEventRecord(e1.e, s));
+            GD_HIP(hipMemcpyAsync(j.rep.p, backup.p, j.rep.bytes, hipMemcpyDeviceToDevice, s));
+            GD_HIP(hipEventSynchronize(e1.e));
         }
         float ms = 0.f;
-        GD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        GD_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
         if (ms < best_ms) {
             best_ms = ms;
-            best = s;
+            best = c;
         }
     }
-    GD_HIP(hipStreamSynchronize(ctx->stream));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    GD_HIP(hipStreamSynchronize(s));
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = best;
@@ -247,85 +392,81 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
             if (int rc = ensure_records(ctx, *js[i], chunk)) return rc;
         table[i] = js[i]->hj;
     }
-    DevBuf jobs_dev, tasks_dev, bases;
+    if (int rc = ensure_group_streams(ctx)) return rc;
+    DevBuf jobs_dev;
     GD_HIP(upload(jobs_dev, table.data(), sizeof(DevJob) * n_jobs, s));
-    GD_HIP(bases.alloc(2 * sizeof(int)));
-    int *step_base = bases.as<int>();
-    int *round_base = step_base + 1;
-
-    // tasks in decreasing cost order so heavy workgroups start first
-    auto make_tasks = [&](int s2max, std::vector<DevTask> &tasks) {
-        tasks.clear();
-        for (int i = 0; i < n_jobs; ++i) build_tasks(*js[i], i, shapes[i], 0, s2max, tasks);
-        std::stable_sort(tasks.begin(), tasks.end(), [&](const DevTask &x, const DevTask &y) {
-            return (long)x.nrep * table[x.job].natom > (long)y.nrep * table[y.job].natom;
-        });
-    };
     std::vector<int> s1_of(n_jobs), round_of(n_jobs);
     for (int i = 0; i < n_jobs; ++i) {
         s1_of[i] = js[i]->steps_done;
         round_of[i] = js[i]->exchange_rounds;
         if (int rc = check_stream_range(*js[i], s1_of[i], (long)s1_of[i] + opts->steps)) return rc;
     }
-    std::vector<DevTask> tasks;
-    make_tasks(chunk, tasks);
-    for (DevTask &tk : tasks) tk.s1 = s1_of[tk.job];
-    GD_HIP(upload(tasks_dev, tasks.data(), sizeof(DevTask) * tasks.size(), s));
+    std::vector<Group> gr = make_groups(js, shapes, gd_ctx::kGroups);
+    GroupsIdle idle{ctx};
     const int zero2[2] = {0, 0};
-    GD_HIP(hipMemcpyAsync(bases.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+    for (Group &g : gr) {
+        group_tasks(g, js, shapes, s1_of, chunk, g.tasks);
+        GD_HIP(upload(g.tasks_dev, g.tasks.data(), sizeof(DevTask) * g.tasks.size(), s));
+        GD_HIP(g.bases.alloc(2 * sizeof(int)));
+        GD_HIP(hipMemcpyAsync(g.bases.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+    }
 
-    auto enqueue_period = [&](int s2) -> int {
-        GD_HIP(launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), tasks_dev.as<DevTask>(), (int)tasks.size(), W, max_rep,
-                         comp, records, s, step_base));
+    // one chunk of a group: Monte Carlo launch, exchange round over the
+    // group's ladders, step advance (no memset, so it can be captured)
+    auto enqueue_period = [&](Group &g, hipStream_t gs, int s2) -> int {
+        GD_HIP(launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), g.tasks_dev.as<DevTask>(), (int)g.tasks.size(), W, max_rep,
+                         comp, records, gs, g.step_base()));
         if (opts->exchange_interval > 0)
-            for (int i = 0; i < n_jobs; ++i)
-                GD_HIP(launch_exchange(jobs_dev.as<DevJob>() + i, js[i]->n_lig * js[i]->n_prt, js[i]->n_lig,
-                                       js[i]->n_tmp, round_of[i], nullptr, s, round_base));
-        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, step_base, s2, round_base);
+            for (const Group::Range &r : g.ranges) {
+                const JobRes &j = *js[r.job];
+                GD_HIP(launch_exchange(jobs_dev.as<DevJob>() + r.job, r.p0 * j.n_lig, (r.p1 - r.p0) * j.n_lig, j.n_lig,
+                                       j.n_tmp, round_of[r.job], nullptr, gs, g.round_base()));
+            }
+        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, gs, g.step_base(), s2, g.round_base());
         GD_HIP(hipGetLastError());
         return GD_OK;
     };
 
     const int n_full = opts->steps / chunk;
     const int rest = opts->steps - n_full * chunk;
-    hipEvent_t e0, e1;
-    GD_HIP(hipEventCreate(&e0));
-    GD_HIP(hipEventCreate(&e1));
-    GD_HIP(hipEventRecord(e0, s));
-    if (n_full > 0) {
-        if (opts->use_graph) {
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            GD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_period(chunk);
-            hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (n_full > 0 && opts->use_graph)
+        for (size_t k = 0; k < gr.size(); ++k) {
+            hipStream_t gs = ctx->group_stream[k];
+            GD_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_period(gr[k], gs, chunk);
+            const hipError_t ce = hipStreamEndCapture(gs, &gr[k].graph);
             if (rc) return rc;
             GD_HIP(ce);
-            GD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            for (int c = 0; c < n_full; ++c) GD_HIP(hipGraphLaunch(exec, s));
-            GD_HIP(hipStreamSynchronize(s));
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-        } else {
-            for (int c = 0; c < n_full; ++c)
-                if (int rc = enqueue_period(chunk)) return rc;
+            GD_HIP(hipGraphInstantiate(&gr[k].exec, gr[k].graph, nullptr, nullptr, 0));
         }
-    }
-    if (rest > 0) {
-        std::vector<DevTask> tail;
-        make_tasks(rest, tail);
-        for (DevTask &tk : tail) tk.s1 = s1_of[tk.job];
-        GD_HIP(upload(tasks_dev, tail.data(), sizeof(DevTask) * tail.size(), s));
-        tasks.swap(tail);
-        GD_HIP(launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), tasks_dev.as<DevTask>(), (int)tasks.size(), W, max_rep,
-                         comp, records, s, step_base));
-    }
-    GD_HIP(hipEventRecord(e1, s));
-    GD_HIP(hipEventSynchronize(e1));
+    Event e0, e1, ef, ej[gd_ctx::kGroups];
+    for (Event *e : {&e0, &e1, &ef}) GD_HIP(hipEventCreate(&e->e));
+    for (Event &e : ej) GD_HIP(hipEventCreateWithFlags(&e.e, hipEventDisableTiming));
+    GD_HIP(hipEventRecord(e0.e, s));
+    if (int rc = fork_groups(ctx, s, gr.size(), ef.e)) return rc;
+    for (int c = 0; c < n_full; ++c)
+        for (size_t k = 0; k < gr.size(); ++k) {
+            if (opts->use_graph)
+                GD_HIP(hipGraphLaunch(gr[k].exec, ctx->group_stream[k]));
+            else if (int rc = enqueue_period(gr[k], ctx->group_stream[k], chunk))
+                return rc;
+        }
+    if (rest > 0)
+        for (size_t k = 0; k < gr.size(); ++k) {
+            Group &g = gr[k];
+            hipStream_t gs = ctx->group_stream[k];
+            group_tasks(g, js, shapes, s1_of, rest, g.tail);
+            // same task count as the chunk's table, so no reallocation while
+            // the group's earlier launches still read it
+            GD_HIP(upload(g.tasks_dev, g.tail.data(), sizeof(DevTask) * g.tail.size(), gs));
+            GD_HIP(launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), g.tasks_dev.as<DevTask>(), (int)g.tail.size(), W,
+                             max_rep, comp, records, gs, g.step_base()));
+        }
+    if (int rc = join_groups(ctx, s, gr.size(), ej)) return rc;
+    GD_HIP(hipEventRecord(e1.e, s));
+    GD_HIP(hipEventSynchronize(e1.e));
     float ms = 0.f;
-    GD_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    GD_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
     if (elapsed_ms) *elapsed_ms = ms;
     for (int i = 0; i < n_jobs; ++i) {
         js[i]->steps_done += opts->steps;

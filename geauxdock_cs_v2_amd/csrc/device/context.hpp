@@ -92,6 +92,10 @@ struct gd_ctx {
     int next_id = 1;
     // launch scratch reused across calls
     gd::DevBuf task_buf, job_buf, pose_rep, pose_move, pose_out, counter;
+    // streams of the batched form's workgroup groups (created on first use,
+    // default priority)
+    static constexpr int kGroups = 2;
+    hipStream_t group_stream[kGroups] = {nullptr, nullptr};
 };
 
 namespace gd {

@@ -141,9 +141,10 @@ hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob
                          int ntask, int waves, bool compensated, const int *pose_rep, const float *pose_move,
                          float *pose_out, hipStream_t stream);
 
-// One exchange round over every ladder of a job.
+// One exchange round over ladders ladder0 .. ladder0 + n_ladder - 1 of a job
+// (ladder p * n_lig + l: ligand conformation l, protein conformation p).
 // round_base (device int, may be null) is added to `round`.
-hipError_t launch_exchange(const DevJob *job_dev, int n_ladder, int n_lig, int n_tmp, int round,
+hipError_t launch_exchange(const DevJob *job_dev, int ladder0, int n_ladder, int n_lig, int n_tmp, int round,
                            int *accept_count, hipStream_t stream, const int *round_base);
 
 } // namespace gd

@@ -15,11 +15,11 @@
 //
 // One step, per slot: proposal move from the injected stream, rotation and
 // translation of the atom, the six pair terms over the slice's points
-// (src/backend_cpu_mic/energy_prt_v1.cpp), cross-slice sum, per-atom
-// hydrophobic restraint, KDE over the atom type's KDE run
-// (energy_kde_v1.cpp) and the MCS rows assigned to the atom's lane
-// (energy_mcs_v3_ell.cpp); then a per-replica sum, normalisation, weighted
-// total, Metropolis accept and record append
+// (src/backend_cpu_mic/energy_prt_v1.cpp), KDE over the atom type's KDE run
+// (energy_kde_v1.cpp) and MCS rows (energy_mcs_v3_ell.cpp), both spread over
+// the slices so that no wave waits on another's gather chain; cross-slice
+// sum and per-atom hydrophobic restraint; then a per-replica sum,
+// normalisation, weighted total, Metropolis accept and record append
 // (kernel_cpp_montecarlo_soa.cpp:333-390) by one thread per replica.
 #include "device.hpp"
 
@@ -78,6 +78,8 @@ struct Slot {
 
 constexpr int kBlock = 64; // points per plain FP32 partial sum
 constexpr int kVals = 9;   // per-slot terms: vdw, ele hi, pmf, psp, hdb, hpc, kde, mcs, ele lo
+constexpr int kRed = 9;    // per-slot partials of a slice: vdw, ele, pmf, psp, hdb, hpc, ele compensation, kde, mcs
+static_assert(kRed <= kVals, "the cross-slice partials share the per-slot terms' LDS region");
 constexpr int kIst = 5;    // ints of replica state per replica in LDS
 
 } // namespace
@@ -264,8 +266,8 @@ __device__ __forceinline__ float wave_min(float v)
 size_t mc_lds_bytes(int waves, int nrep)
 {
     const int lanes = waves * 64;
-    // one region for the cross-slice partials (7 floats per slot and slice
-    // beyond the first) and, after a barrier, the per-slot terms (kVals
+    // one region for the cross-slice partials (kRed floats per slot and
+    // slice beyond the first) and, after a barrier, the per-slot terms (kVals
     // floats per slot); then rsum, st, prop (16 + 16 + 8 floats per replica)
     // and kIst ints per replica
     return sizeof(float) * ((size_t)lanes * kVals + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * kIst;
@@ -302,7 +304,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     const int a = active ? slot - g * NL : 0;
     const int lanes = W * 64;
 
-    float *red = lds;                          // [(S-1) * SW*64 * 7], then reused:
+    float *red = lds;                          // [(S-1) * SW*64 * kRed], then reused:
     float *vals = lds;                         // [SW*64 * kVals]
     float *rsum = lds + (size_t)lanes * kVals; // [R][16] compensated term sums
     float *st = rsum + (size_t)R * 16;     // [R][16] movematrix[6], energy[10]
@@ -477,56 +479,35 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             acc_add<KAHAN>(epsp, cpsp, ss);
         }
 
-        // ---- cross-slice sum ----
-        if (S > 1 && slice > 0) {
-            float *r = red + ((size_t)(slice - 1) * SW * 64 + slot) * 7;
-            r[0] = evdw;
-            r[1] = eele;
-            r[2] = epmf;
-            r[3] = epsp;
-            r[4] = ehdb;
-            r[5] = ehpc;
-            r[6] = cele; // the electrostatic sum travels with its compensation
-        }
-        __syncthreads();
-        if (slice == 0 && active) {
-            for (int s = 1; s < S; ++s) {
-                const float *r = red + ((size_t)(s - 1) * SW * 64 + slot) * 7;
-                acc_add<KAHAN>(evdw, cvdw, r[0]);
-                acc_add<KAHAN>(eele, cele, r[1]);
-                acc_add<KAHAN>(eele, cele, -r[6]);
-                acc_add<KAHAN>(epmf, cpmf, r[2]);
-                acc_add<KAHAN>(epsp, cpsp, r[3]);
-                acc_add<KAHAN>(ehdb, chdb, r[4]);
-                acc_add<KAHAN>(ehpc, chpc, r[5]);
+        // KDE over the atom type's run of the type-sorted table
+        // (energy_kde_v1.cpp:1-77): slice s takes points kr.x + s, kr.x + s + S, ...
+        // MCS rows over the file-order atom set (energy_mcs_v3_ell.cpp):
+        // worker a * S + s of the replica takes rows a * S + s, + S * N_L, ...,
+        // so the rows of a small table still fall on every slice
+        float ks = 0.f, mcs = 0.f;
+        const int2 kr = J.krange[a];
+        if (active) {
+            int k = kr.x + slice;
+            for (; k + S < kr.y; k += 2 * S) {
+                const float4 K0 = J.kde[k], K1 = J.kde[k + S];
+                const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
+                const float bx = me.x - K1.x, by = me.y - K1.y, bz = me.z - K1.z;
+                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
+                ks += fast_exp(sc.kde2 * fmaf(bx, bx, fmaf(by, by, bz * bz)));
             }
-        }
-        __syncthreads(); // the partials region is reused for the per-slot terms
-        if (slice == 0 && active) {
-            // hydrophobic restraint of the atom (energy_prt_v1.cpp:154-155)
-            const float hpc2 = (ehpc - sc.hpl0[tl]) / sc.hpl1[tl];
-            const float hpc_atom = 0.5f * hpc2 * hpc2 - sc.hpl2[tl];
-
-            // KDE over the atom type's run of the type-sorted table
-            // (energy_kde_v1.cpp:1-77)
-            const int2 kr = J.krange[a];
-            float ks = 0.f;
-            for (int k = kr.x; k < kr.y; ++k) {
-                const float4 K4 = J.kde[k];
-                const float dx = me.x - K4.x, dy = me.y - K4.y, dz = me.z - K4.z;
-                ks += fast_exp(sc.kde2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+            if (k < kr.y) {
+                const float4 K0 = J.kde[k];
+                const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
+                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
             }
-            const float kde_atom = kr.y > kr.x ? ks / (float)(kr.y - kr.x) : 0.0f;
-
-            // MCS rows a, a + N_L, ... of this replica over the file-order
-            // atom set (energy_mcs_v3_ell.cpp)
-            float mcs = 0.f;
-            for (int row = a; row < J.nrow; row += NL) {
+            const float4 *lig2 = J.lig2 + (size_t)idx_lig * NL;
+            for (int row = a * S + slice; row < J.nrow; row += S * NL) {
                 const int rb = J.mrow[row], re = J.mrow[row + 1];
                 float s = 0.f;
-                for (int k = rb; k < re; ++k) {
-                    const float4 E = J.ment[k];
-                    const float4 X = J.lig2[(size_t)idx_lig * NL + __float_as_int(E.w)];
+#pragma unroll 4
+                for (int k2 = rb; k2 < re; ++k2) {
+                    const float4 E = J.ment[k2];
+                    const float4 X = lig2[__float_as_int(E.w)];
                     const float ax = r00 * X.x + r01 * X.y + r02 * X.z + tx;
                     const float ay = r10 * X.x + r11 * X.y + r12 * X.z + ty;
                     const float az = r20 * X.x + r21 * X.y + r22 * X.z + tz;
@@ -535,6 +516,42 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 }
                 mcs += J.mtcc[row] * sqrtf(s / (float)(re - rb));
             }
+        }
+
+        // ---- cross-slice sum ----
+        if (S > 1 && slice > 0) {
+            float *r = red + ((size_t)(slice - 1) * SW * 64 + slot) * kRed;
+            r[0] = evdw;
+            r[1] = eele;
+            r[2] = epmf;
+            r[3] = epsp;
+            r[4] = ehdb;
+            r[5] = ehpc;
+            r[6] = cele; // the electrostatic sum travels with its compensation
+            r[7] = ks;
+            r[8] = mcs;
+        }
+        __syncthreads();
+        if (slice == 0 && active) {
+            for (int s = 1; s < S; ++s) {
+                const float *r = red + ((size_t)(s - 1) * SW * 64 + slot) * kRed;
+                acc_add<KAHAN>(evdw, cvdw, r[0]);
+                acc_add<KAHAN>(eele, cele, r[1]);
+                acc_add<KAHAN>(eele, cele, -r[6]);
+                acc_add<KAHAN>(epmf, cpmf, r[2]);
+                acc_add<KAHAN>(epsp, cpsp, r[3]);
+                acc_add<KAHAN>(ehdb, chdb, r[4]);
+                acc_add<KAHAN>(ehpc, chpc, r[5]);
+                ks += r[7];
+                mcs += r[8];
+            }
+        }
+        __syncthreads(); // the partials region is reused for the per-slot terms
+        if (slice == 0 && active) {
+            // hydrophobic restraint of the atom (energy_prt_v1.cpp:154-155)
+            const float hpc2 = (ehpc - sc.hpl0[tl]) / sc.hpl1[tl];
+            const float hpc_atom = 0.5f * hpc2 * hpc2 - sc.hpl2[tl];
+            const float kde_atom = kr.y > kr.x ? ks / (float)(kr.y - kr.x) : 0.0f;
             float *v = vals + (size_t)slot * kVals;
             v[0] = evdw * sc.inv_lj0;
             // electrostatics lc / el1 (eele - cele) as hi + lo: the atoms'
@@ -645,11 +662,12 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
 
 // One thread per ladder (ligand conformation l, protein conformation p):
 // temperature replica exchange, see gd_exchange in geauxdock_amd.h.
-__global__ void exchange_kernel(const DevJob *__restrict__ job, int n_ladder, int n_lig, int n_tmp, int round,
-                                const int *__restrict__ round_base, int *__restrict__ accept_count)
+__global__ void exchange_kernel(const DevJob *__restrict__ job, int ladder0, int n_ladder, int n_lig, int n_tmp,
+                                int round, const int *__restrict__ round_base, int *__restrict__ accept_count)
 {
-    const int L = blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= n_ladder) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_ladder) return;
+    const int L = ladder0 + i;
     const int rnd = round + (round_base ? *round_base : 0);
     const DevJob &J = *job;
     const int p = L / n_lig, l = L - p * n_lig;
@@ -718,13 +736,13 @@ hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob
     return hipGetLastError();
 }
 
-hipError_t launch_exchange(const DevJob *job_dev, int n_ladder, int n_lig, int n_tmp, int round, int *accept_count,
-                           hipStream_t stream, const int *round_base)
+hipError_t launch_exchange(const DevJob *job_dev, int ladder0, int n_ladder, int n_lig, int n_tmp, int round,
+                           int *accept_count, hipStream_t stream, const int *round_base)
 {
     if (n_ladder <= 0 || n_tmp < 2) return hipSuccess;
     const int bs = 64;
-    hipLaunchKernelGGL(exchange_kernel, dim3((n_ladder + bs - 1) / bs), dim3(bs), 0, stream, job_dev, n_ladder, n_lig,
-                       n_tmp, round, round_base, accept_count);
+    hipLaunchKernelGGL(exchange_kernel, dim3((n_ladder + bs - 1) / bs), dim3(bs), 0, stream, job_dev, ladder0, n_ladder,
+                       n_lig, n_tmp, round, round_base, accept_count);
     return hipGetLastError();
 }
 

@@ -210,7 +210,7 @@ void gd_server::loop()
                 const int after = q.done + std::min(opts.chunk_steps, q.steps - q.done);
                 if (after % opts.exchange_interval == 0) {
                     JobRes &j = ctx->jobs.at(q.job_id);
-                    e = launch_exchange(jobs_dev.as<DevJob>() + i, j.n_lig * j.n_prt, j.n_lig, j.n_tmp, q.rounds,
+                    e = launch_exchange(jobs_dev.as<DevJob>() + i, 0, j.n_lig * j.n_prt, j.n_lig, j.n_tmp, q.rounds,
                                         nullptr, ctx->stream, nullptr);
                     exch[i] = 1;
                 }
