@@ -85,6 +85,13 @@ int ensure_records(gd_ctx *ctx, JobRes &j, int cap)
 // amplify.  The shift is done in double and rounded once.
 float local(float x, float origin) { return (float)((double)x - (double)origin); }
 
+int ensure_group_streams(gd_ctx *ctx)
+{
+    for (hipStream_t &g : ctx->group_stream)
+        if (!g) GD_HIP(hipStreamCreateWithFlags(&g, hipStreamNonBlocking));
+    return GD_OK;
+}
+
 int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
               int waves, int max_rep, bool compensated, bool records)
 {

@@ -207,13 +207,6 @@ void group_tasks(const Group &g, const std::vector<JobRes *> &js, const std::vec
     });
 }
 
-int ensure_group_streams(gd_ctx *ctx)
-{
-    for (hipStream_t &g : ctx->group_stream)
-        if (!g) GD_HIP(hipStreamCreateWithFlags(&g, hipStreamNonBlocking));
-    return GD_OK;
-}
-
 // order the group streams after the work on s (fork), or s after the group
 // streams (join)
 int fork_groups(gd_ctx *ctx, hipStream_t s, size_t n, hipEvent_t ev)

@@ -105,4 +105,6 @@ int ensure_records(gd_ctx *ctx, JobRes &j, int cap);
 int check_stream_range(const JobRes &j, long s1, long s_end);
 int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
               int waves, int max_rep, bool compensated, bool records);
+// creates the context's group streams on first use
+int ensure_group_streams(gd_ctx *ctx);
 } // namespace gd

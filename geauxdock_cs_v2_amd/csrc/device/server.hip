@@ -3,11 +3,15 @@
 //
 // Requests are queued by gd_server_submit.  A worker thread admits queued
 // requests in arrival order while the device-memory budget and the slot count
-// allow (admission by HBM footprint), then advances every admitted request by
-// one chunk of Monte Carlo steps in a single fused launch -- requests that
-// arrived while others were on the GPU join at the next chunk boundary, each
-// at its own step counter.  Finished requests are read back, their device
-// memory freed, and the freed room lets queued requests in.
+// allow (admission by HBM footprint) and places each in one of two lanes.  A
+// lane advances all of its requests by one chunk of Monte Carlo steps in a
+// single fused launch on its own stream; the two lanes run side by side and
+// are relaunched independently, so the tail of one lane's launch (its last,
+// partly filled round of workgroups) overlaps the other lane's work.
+// Requests that arrive while the lanes are busy join an idle lane at its
+// next chunk boundary, each at its own step counter.  Finished requests are
+// read back, their device memory freed, and the freed room lets queued
+// requests in.
 //
 // Every request keeps the workgroup shape its ligand size alone decides and
 // its own keyed random stream, so its result is the same whether it runs
@@ -34,6 +38,7 @@ namespace gd {
 namespace {
 
 constexpr int kServerWaves = 12;
+constexpr int kLanes = gd_ctx::kGroups;
 // a request's shape depends on the request alone (so its result does not
 // depend on the mix): the shape that would fill the GPU best were it alone
 
@@ -83,6 +88,18 @@ struct JobCopy {
     }
 };
 
+// one chunk launch of a lane: its requests, their tables and step counts
+struct Lane {
+    std::vector<int> ids;
+    std::vector<int> s2, exch; // steps of the chunk, exchange round after it
+    std::vector<DevJob> table;
+    std::vector<DevTask> tasks;
+    DevBuf jobs_dev, tasks_dev;
+    hipEvent_t done = nullptr;
+    hipError_t err = hipSuccess;
+    bool busy = false;
+};
+
 struct Request {
     int id = 0;
     JobCopy job;
@@ -92,6 +109,7 @@ struct Request {
     int job_id = -1;
     int done = 0;
     int rounds = 0;
+    int lane = 0;
     bool cancel = false;
     size_t footprint = 0;
     double t_submit = 0, t_admit = -1, t_done = -1;
@@ -117,7 +135,7 @@ struct gd_server {
     bool stop = false;
     std::thread worker;
     std::chrono::steady_clock::time_point t0;
-    DevBuf jobs_dev, tasks_dev;
+    Lane lanes[kLanes];
 
     double now() const
     {
@@ -126,7 +144,32 @@ struct gd_server {
     void loop();
     void admit_locked();
     void finish_locked(Request &q, int status);
+    double lane_load_locked(int k) const;
+    void rebalance_locked();
+    void prepare_locked(int k);
+    void launch(int k);
+    void complete_locked(int k);
+    bool any_busy() const
+    {
+        for (const Lane &l : lanes)
+            if (l.busy) return true;
+        return false;
+    }
 };
+
+// remaining pair evaluations of the lane's requests
+double gd_server::lane_load_locked(int k) const
+{
+    double w = 0.0;
+    for (int id : active) {
+        const Request &q = reqs.at(id);
+        if (q.lane == k) {
+            const JobRes &j = ctx->jobs.at(q.job_id);
+            w += (double)j.n_rep * j.natom * (q.steps - q.done);
+        }
+    }
+    return w;
+}
 
 void gd_server::finish_locked(Request &q, int status)
 {
@@ -160,88 +203,198 @@ void gd_server::admit_locked()
         q.job_id = jid;
         used += q.footprint;
         q.t_admit = now();
+        q.lane = lane_load_locked(1) < lane_load_locked(0) ? 1 : 0;
         active.push_back(q.id);
     }
+}
+
+// Before the idle lanes are gathered: an idle lane left with no requests
+// takes about half the load of the idle lane with the most requests, if that
+// lane has two or more (both idle, so no request in flight moves; a lane
+// never gives away its last request).  Moving a request changes no result.
+void gd_server::rebalance_locked()
+{
+    int count[kLanes] = {};
+    for (int id : active) ++count[reqs.at(id).lane];
+    for (int y = 0; y < kLanes; ++y) {
+        if (lanes[y].busy || count[y] != 0) continue;
+        int k = -1;
+        for (int x = 0; x < kLanes; ++x)
+            if (x != y && !lanes[x].busy && count[x] >= 2 && (k < 0 || count[x] > count[k])) k = x;
+        if (k < 0) continue;
+        const double half = 0.5 * lane_load_locked(k);
+        double moved = 0.0;
+        for (int id : active) {
+            Request &q = reqs.at(id);
+            if (q.lane != k || moved >= half || count[k] <= 1) continue;
+            const JobRes &j = ctx->jobs.at(q.job_id);
+            const double w = (double)j.n_rep * j.natom * (q.steps - q.done);
+            if (moved + w > half && moved > 0.0) continue;
+            q.lane = y;
+            moved += w;
+            --count[k];
+            ++count[y];
+        }
+    }
+}
+
+// Gather the chunk of idle lane k.
+void gd_server::prepare_locked(int k)
+{
+    Lane &L = lanes[k];
+    L.ids.clear();
+    L.s2.clear();
+    L.exch.clear();
+    L.table.clear();
+    L.tasks.clear();
+    L.err = hipSuccess;
+    for (int id : active) {
+        Request &q = reqs.at(id);
+        if (q.lane != k) continue;
+        JobRes &j = ctx->jobs.at(q.job_id);
+        const int s2 = std::min(opts.chunk_steps, q.steps - q.done);
+        const int after = q.done + s2;
+        L.ids.push_back(id);
+        L.s2.push_back(s2);
+        L.exch.push_back(opts.exchange_interval > 0 && after % opts.exchange_interval == 0 ? 1 : 0);
+        build_tasks(j, (int)L.table.size(),
+                    heuristic_shape(j.natom, kServerWaves, j.n_tmp * j.n_lig, j.n_prt, 2 * ctx->cu_count), q.done, s2,
+                    L.tasks);
+        L.table.push_back(j.hj);
+    }
+    std::stable_sort(L.tasks.begin(), L.tasks.end(), [&](const DevTask &x, const DevTask &y) {
+        return (long)x.nrep * L.table[x.job].natom * x.s2max > (long)y.nrep * L.table[y.job].natom * y.s2max;
+    });
+}
+
+// Enqueue lane k's chunk on its stream (outside the lock: the lane's tables
+// belong to the worker thread while it is busy).
+void gd_server::launch(int k)
+{
+    Lane &L = lanes[k];
+    hipStream_t gs = ctx->group_stream[k];
+    int max_rep = 1;
+    for (const DevTask &t : L.tasks) max_rep = std::max(max_rep, t.nrep);
+    const TargetRes &t = ctx->targets.at(target);
+    hipError_t e = upload(L.jobs_dev, L.table.data(), sizeof(DevJob) * L.table.size(), gs);
+    if (e == hipSuccess) e = upload(L.tasks_dev, L.tasks.data(), sizeof(DevTask) * L.tasks.size(), gs);
+    if (e == hipSuccess)
+        e = launch_mc(t.dt, t.sc, L.jobs_dev.as<DevJob>(), L.tasks_dev.as<DevTask>(), (int)L.tasks.size(), kServerWaves,
+                      max_rep, true, false, gs, nullptr);
+    for (size_t i = 0; i < L.ids.size() && e == hipSuccess; ++i) {
+        if (!L.exch[i]) continue;
+        int rounds, n_lig, n_prt, n_tmp;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            const Request &q = reqs.at(L.ids[i]);
+            const JobRes &j = ctx->jobs.at(q.job_id);
+            rounds = q.rounds, n_lig = j.n_lig, n_prt = j.n_prt, n_tmp = j.n_tmp;
+        }
+        e = launch_exchange(L.jobs_dev.as<DevJob>() + i, 0, n_lig * n_prt, n_lig, n_tmp, rounds, nullptr, gs, nullptr);
+    }
+    if (e == hipSuccess) e = hipEventRecord(L.done, gs);
+    if (e != hipSuccess) (void)hipStreamSynchronize(gs);
+    L.err = e;
+    L.busy = true;
+}
+
+// Bookkeeping of lane k's finished chunk.
+void gd_server::complete_locked(int k)
+{
+    Lane &L = lanes[k];
+    const hipError_t e = L.err;
+    for (size_t i = 0; i < L.ids.size(); ++i) {
+        Request &q = reqs.at(L.ids[i]);
+        if (e != hipSuccess) {
+            fail(GD_ERR_DEVICE, std::string("serving launch failed: ") + hipGetErrorString(e));
+            finish_locked(q, GD_ERR_DEVICE);
+            continue;
+        }
+        q.done += L.s2[i];
+        q.rounds += L.exch[i];
+        if (q.cancel) {
+            finish_locked(q, GD_ERR_STATE);
+        } else if (q.done >= q.steps) {
+            q.result.resize(ctx->jobs.at(q.job_id).n_rep);
+            const int rc = gd_read_replicas(ctx, q.job_id, q.result.data());
+            finish_locked(q, rc);
+        }
+    }
+    std::vector<int> still;
+    for (int id : active)
+        if (reqs.at(id).job_id >= 0) still.push_back(id);
+    active.swap(still);
+    L.ids.clear();
+    L.busy = false;
+    cv.notify_all();
 }
 
 void gd_server::loop()
 {
     (void)hipSetDevice(ctx->device);
-    std::vector<DevJob> table;
-    std::vector<DevTask> tasks;
-    std::vector<int> ids;
+    bool ok = ensure_group_streams(ctx) == GD_OK;
+    for (Lane &l : lanes) ok = ok && hipEventCreateWithFlags(&l.done, hipEventDisableTiming) == hipSuccess;
     while (true) {
+        bool launch_k[kLanes] = {};
         {
             std::unique_lock<std::mutex> lk(mu);
+            if (!ok) { // no streams: every request fails
+                for (int id : active) finish_locked(reqs.at(id), GD_ERR_DEVICE);
+                active.clear();
+                while (!pending.empty()) {
+                    finish_locked(reqs.at(pending.front()), GD_ERR_DEVICE);
+                    pending.pop_front();
+                }
+                cv.notify_all();
+                cv.wait(lk, [this] { return stop || !pending.empty(); });
+                if (stop) break;
+                continue;
+            }
             admit_locked();
             cv.notify_all();
-            while (active.empty() && !stop && (pending.empty() || used + reqs.at(pending.front()).footprint > opts.hbm_budget)) {
+            while (!stop && active.empty() && (pending.empty() || used + reqs.at(pending.front()).footprint > opts.hbm_budget)) {
                 cv.wait(lk);
                 admit_locked();
             }
-            if (stop) return;
-            ids = active;
-            table.clear();
-            tasks.clear();
-            for (size_t i = 0; i < ids.size(); ++i) {
-                Request &q = reqs.at(ids[i]);
-                JobRes &j = ctx->jobs.at(q.job_id);
-                table.push_back(j.hj);
-                const int s2 = std::min(opts.chunk_steps, q.steps - q.done);
-                build_tasks(j, (int)i, heuristic_shape(j.natom, kServerWaves, j.n_tmp * j.n_lig, j.n_prt, 2 * ctx->cu_count), q.done, s2, tasks);
-            }
-        }
-        int max_rep = 1;
-        for (const DevTask &t : tasks) max_rep = std::max(max_rep, t.nrep);
-        std::stable_sort(tasks.begin(), tasks.end(), [&](const DevTask &x, const DevTask &y) {
-            return (long)x.nrep * table[x.job].natom * x.s2max > (long)y.nrep * table[y.job].natom * y.s2max;
-        });
-        const TargetRes &t = ctx->targets.at(target);
-        hipError_t e = upload(jobs_dev, table.data(), sizeof(DevJob) * table.size(), ctx->stream);
-        if (e == hipSuccess) e = upload(tasks_dev, tasks.data(), sizeof(DevTask) * tasks.size(), ctx->stream);
-        if (e == hipSuccess)
-            e = launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), tasks_dev.as<DevTask>(), (int)tasks.size(), kServerWaves,
-                          max_rep, true, false, ctx->stream, nullptr);
-        std::vector<int> exch(ids.size(), 0);
-        if (e == hipSuccess && opts.exchange_interval > 0) {
-            std::lock_guard<std::mutex> lk(mu);
-            for (size_t i = 0; i < ids.size() && e == hipSuccess; ++i) {
-                Request &q = reqs.at(ids[i]);
-                const int after = q.done + std::min(opts.chunk_steps, q.steps - q.done);
-                if (after % opts.exchange_interval == 0) {
-                    JobRes &j = ctx->jobs.at(q.job_id);
-                    e = launch_exchange(jobs_dev.as<DevJob>() + i, 0, j.n_lig * j.n_prt, j.n_lig, j.n_tmp, q.rounds,
-                                        nullptr, ctx->stream, nullptr);
-                    exch[i] = 1;
+            if (stop) break;
+            rebalance_locked();
+            for (int k = 0; k < kLanes; ++k)
+                if (!lanes[k].busy) {
+                    prepare_locked(k);
+                    launch_k[k] = !lanes[k].ids.empty();
                 }
-            }
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-
-        std::lock_guard<std::mutex> lk(mu);
-        std::vector<int> still;
-        for (size_t i = 0; i < ids.size(); ++i) {
-            Request &q = reqs.at(ids[i]);
-            if (e != hipSuccess) {
-                fail(GD_ERR_DEVICE, std::string("serving launch failed: ") + hipGetErrorString(e));
-                finish_locked(q, GD_ERR_DEVICE);
-                continue;
+        for (int k = 0; k < kLanes; ++k)
+            if (launch_k[k]) launch(k);
+        // wait for a lane to finish, or for a request an idle lane can take
+        while (any_busy()) {
+            bool finished = false;
+            for (int k = 0; k < kLanes; ++k) {
+                Lane &L = lanes[k];
+                if (!L.busy) continue;
+                const hipError_t q = L.err == hipSuccess ? hipEventQuery(L.done) : L.err;
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) L.err = q;
+                std::lock_guard<std::mutex> lk(mu);
+                complete_locked(k);
+                finished = true;
             }
-            q.done += std::min(opts.chunk_steps, q.steps - q.done);
-            q.rounds += exch[i];
-            if (q.cancel) {
-                finish_locked(q, GD_ERR_STATE);
-            } else if (q.done >= q.steps) {
-                q.result.resize(ctx->jobs.at(q.job_id).n_rep);
-                const int rc = gd_read_replicas(ctx, q.job_id, q.result.data());
-                finish_locked(q, rc);
-            } else {
-                still.push_back(q.id);
+            if (finished) break;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                bool idle = false;
+                for (const Lane &l : lanes) idle |= !l.busy;
+                if (!stop && idle && !pending.empty() && used + reqs.at(pending.front()).footprint <= opts.hbm_budget &&
+                    (int)active.size() < opts.max_active_jobs)
+                    break;
             }
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
-        active.swap(still);
-        cv.notify_all();
     }
+    for (int k = 0; k < kLanes; ++k)
+        if (ctx->group_stream[k]) (void)hipStreamSynchronize(ctx->group_stream[k]);
+    for (Lane &l : lanes)
+        if (l.done) (void)hipEventDestroy(l.done);
 }
 
 extern "C" {
