@@ -13,8 +13,10 @@ Beside it, each in its own section of the JSON line:
   plain     the plain form (one workgroup per replica, one launch per
             100-step chunk, record read-back per chunk) over the same steps;
   roofline  the Monte Carlo kernel's algorithmic FLOP rate: FLOP per pose x
-            poses per launch / the average launch time of the timed region
-            (hipEvents on the launch stream); traffic from the committed
+            poses per 100-step chunk / the average chunk time of the timed
+            region (hipEvents around the chunk sequence; a chunk is the
+            kernel's dispatches for the two workgroup groups, which run side
+            by side on two streams); traffic per chunk from the committed
             rocprofv3 counter summary;
   screen    batched screening: 1,024 ligands (lognormal 12-96 atoms, 8
             conformations, 8 temperatures; ~393k replicas) against one
@@ -80,11 +82,11 @@ def cpu_baseline(cx, target_seconds):
 
 
 def traffic_from_summary():
-    """HBM bytes per 100-step launch of the headline kernel, as the committed
+    """HBM bytes per 100-step chunk of the headline kernel, as the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary states it (None if absent)."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
-            m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+            m = re.search(r"^bytes_per_chunk:\s*([0-9.eE+]+)", f.read(), re.M)
         return float(m.group(1)) if m else None
     except OSError:
         return None
@@ -112,12 +114,14 @@ def run_c2(args, dev, perf, synth):
     wall = time.perf_counter() - t0
     shape_used = dev.job_shape(j)
     value = n_rep * args.steps / wall
-    launches = math.ceil(args.steps / chunk)
-    launch_ms = dev_ms / launches  # one Monte Carlo launch per chunk (exchange and step advance are ~10 us)
-    steps_per_launch = args.steps / launches
-    achieved = f_pose * n_rep * steps_per_launch / (launch_ms * 1e-3) / 1e12
-    log(f"c2 batched: {args.steps} steps in {wall:.3f} s wall ({dev_ms:.1f} ms device, {launches} launches "
-        f"of {launch_ms:.2f} ms), {value:.4g} poses/s, {achieved:.1f} TFLOP/s")
+    chunks = math.ceil(args.steps / chunk)
+    # a chunk: the Monte Carlo kernel over every replica (one dispatch per
+    # workgroup group, side by side), then exchange and step advance (~10 us)
+    chunk_ms = dev_ms / chunks
+    steps_per_chunk = args.steps / chunks
+    achieved = f_pose * n_rep * steps_per_chunk / (chunk_ms * 1e-3) / 1e12
+    log(f"c2 batched: {args.steps} steps in {wall:.3f} s wall ({dev_ms:.1f} ms device, {chunks} chunks "
+        f"of {chunk_ms:.2f} ms), {value:.4g} poses/s, {achieved:.1f} TFLOP/s")
     dev.free_job(j)
 
     plain_value = None
@@ -140,12 +144,14 @@ def run_c2(args, dev, perf, synth):
     traffic = traffic_from_summary()
     roof = {"bound": "valu", "achieved": achieved, "peak": perf.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / perf.FP32_PEAK_TFLOPS, "traffic": traffic,
-            "kernel": "mc_kernel", "launch_ms": launch_ms, "steps_per_launch": steps_per_launch,
+            "kernel": "mc_kernel", "chunk_ms": chunk_ms, "steps_per_chunk": steps_per_chunk,
+            "unit_of_work": "one 100-step chunk of all replicas: the kernel's dispatches for the two "
+                            "workgroup groups, run side by side on two streams",
             "flop_per_pose": f_pose, "fma_issue_peak": perf.FP32_FMA_TFLOPS,
             "shape": {"waves": shape_used[0], "slot_waves": shape_used[1], "point_slices": shape_used[2],
                       "replicas_per_group": shape_used[3], "chosen_by": "given" if args.shape else "autotuner"}}
     if traffic:
-        roof["hbm_gbs"] = traffic / (launch_ms * 1e-3) / 1e9
+        roof["hbm_gbs"] = traffic / (chunk_ms * 1e-3) / 1e9
         roof["hbm_frac"] = roof["hbm_gbs"] / perf.HBM_PEAK_GBS
         roof["traffic_source"] = os.path.relpath(TRAFFIC_SUMMARY, ROOT)
     workload = (f"c2: ligand {cx.size['lig_natom']} atoms x {cx.size['n_lig']} conf, protein "

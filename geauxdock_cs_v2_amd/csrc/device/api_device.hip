@@ -298,6 +298,33 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             }
         far_d += 0.05;
     }
+    // Very far field.  The LJ denominator is 1 + e, e = X (lj_a + lj_b d2),
+    // X = c p1a / d^9; e falls with d (lj_a, lj_b >= 0).  Beyond vfar_d,
+    // e <= 3e-3 for every type pair, so 1 - e + e^2 equals 1 / (1 + e) to
+    // within e^3 < 2^-25 and the kernel needs no reciprocal there.
+    double vfar_d = far_d;
+    {
+        const double kEps = 3.0e-3;
+        const double ljc_ = ep.lj0 != 0.0f ? (double)ep.lj0 : 1.0;
+        const double a = ep.lj0 / ljc_, b = ep.lj0 / ljc_ * ep.lj1;
+        if (a < 0.0 || b < 0.0) {
+            vfar_d = 1e30; // softening that does not fall with d: no very far field
+        } else {
+            for (int tp = 0; tp < GD_MAXTP1; ++tp)
+                for (int tl = 0; tl < GD_MAXTP2; ++tl) {
+                    const double p1 = (double)(float)(ljc_ * ep.p1a[tl][tp]);
+                    auto e_at = [&](double d) { return p1 * (a + b * d * d) / std::pow(d, 9); };
+                    if (p1 <= 0.0 || e_at(vfar_d) <= kEps) continue;
+                    double lo = vfar_d, hi = 4096.0;
+                    for (int it = 0; it < 80; ++it) {
+                        const double m = 0.5 * (lo + hi);
+                        (e_at(m) > kEps ? lo : hi) = m;
+                    }
+                    vfar_d = hi;
+                }
+            vfar_d += 0.05;
+        }
+    }
     std::vector<int> ladder(seg.size() * kFarRungs), ladder_in(seg.size() * kFarRungs);
     for (size_t g = 0; g < seg.size(); ++g)
         for (int k = 0; k < kFarRungs; ++k) {
@@ -321,6 +348,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.ladder = t.ladder.as<int>();
     t.dt.ladder_in = t.ladder_in.as<int>();
     t.dt.far_d = (float)far_d;
+    t.dt.vfar_d = (float)std::min(vfar_d, 1e30);
     t.dt.pt = t.pt.as<float4>();
     t.dt.pa = t.pa.as<float2>();
     t.dt.pp = t.pp.as<float>();

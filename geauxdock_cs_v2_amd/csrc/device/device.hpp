@@ -81,6 +81,7 @@ struct DevTarget {
     const int *ladder; // [nseg][kFarRungs] first point of the segment at radius >= rung
     const int *ladder_in; // [nseg][kFarRungs] end of the segment's leading points with largest radius <= rung
     float far_d;       // distance beyond which only LJ and 1/d electrostatics vary
+    float vfar_d;      // distance beyond which 1 / (1 + e) of the soft-core LJ is 1 - e + e^2 to float rounding
     int npoint, npad, nseg, nconf;
 };
 

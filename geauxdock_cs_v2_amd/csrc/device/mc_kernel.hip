@@ -247,6 +247,45 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int p
     s.p = fmaf(q.pmf1, (float)(pe - pb), s.p);
 }
 
+// Very far field: points at least vfar_d from every atom of the wave, where
+// the LJ softening e = X (lj_a + lj_b d2) is at most 3e-3 for every type
+// pair (DevTarget::vfar_d): n / (1 + e) = n (1 - e + e^2) to float rounding,
+// so the pair needs one transcendental (1/d) instead of two.
+__device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, int pb, int pe, const Slot &me,
+                                                const PairParams &q, const DevScalars &sc, Sums &s)
+{
+    const cfloat *ptc = (const cfloat *)ptc_;
+    int p = pb;
+    for (; p + 4 <= pe; p += 4) {
+        float ge = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 P4 = ld_point(ptc, p + j);
+            const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float rinv = __builtin_amdgcn_rsqf(d2);
+            const float r2 = rinv * rinv;
+            const float r3 = r2 * rinv;
+            const float r6 = r3 * r3;
+            const float X = q.p1a * (r6 * r3);
+            const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
+            const float n = fmaf(-q.p2a, r6, X);
+            s.v += fmaf(-n, fmaf(-e, e, e), n); // n - n (e - e^2)
+            ge = fmaf(P4.w, rinv, ge);
+        }
+        kahan_add(s.e, s.ec, ge);
+    }
+    for (; p < pe; ++p) {
+        const float4 P4 = ld_point(ptc, p);
+        const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float rinv = __builtin_amdgcn_rsqf(d2);
+        lj_term(d2, rinv, q, sc, s);
+        kahan_add(s.e, s.ec, P4.w * rinv);
+    }
+    s.p = fmaf(q.pmf1, (float)(pe - pb), s.p);
+}
+
 __device__ __forceinline__ float wave_max(float v)
 {
 #pragma unroll
@@ -405,6 +444,12 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         const int kf = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho_out + tg.far_d - kFarR0) * (1.0f / kFarDr))));
         const int ki = __builtin_amdgcn_readfirstlane(
             min(kFarRungs - 1, (int)floorf((rho_in - tg.far_d - kFarR0) * (1.0f / kFarDr))));
+        // the same rungs for the very far field (vfar_d >= far_d, so kv >= kf
+        // and kvi <= ki; vfar_d may be huge: clamp before converting)
+        const int kv = __builtin_amdgcn_readfirstlane(
+            (int)fminf(fmaxf(ceilf((rho_out + tg.vfar_d - kFarR0) * (1.0f / kFarDr)), 0.0f), (float)kFarRungs));
+        const int kvi = __builtin_amdgcn_readfirstlane(
+            (int)fmaxf(fminf(floorf((rho_in - tg.vfar_d - kFarR0) * (1.0f / kFarDr)), (float)(kFarRungs - 1)), -1.0f));
         const cint *ladder = (const cint *)tg.ladder;
         const cint *ladder_in = (const cint *)tg.ladder_in;
         for (int si = 0; si < tg.nseg; ++si) {
@@ -413,10 +458,15 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             if (b >= e) continue;
             const int ie = ki >= 0 ? ladder_in[si * kFarRungs + ki] : sg.y;
             const int ob = kf < kFarRungs ? ladder[si * kFarRungs + kf] : sg.z;
-            // far [b, m1), near [m1, mid), far [mid, e); the two far runs
-            // cannot share a point, so ie <= ob
-            const int m1 = min(max(ie, b), e);
+            const int ive = kvi >= 0 ? ladder_in[si * kFarRungs + kvi] : sg.y;
+            const int ovb = kv < kFarRungs ? ladder[si * kFarRungs + kv] : sg.z;
+            // very far [b, v1), far [v1, m1), near [m1, mid), far [mid, v2),
+            // very far [v2, e); the inner and outer runs cannot share a
+            // point (ive <= ie <= ob <= ovb), the clamps keep the order
+            const int v1 = min(max(ive, b), e);
+            const int m1 = min(max(ie, v1), e);
             const int mid = min(max(ob, m1), e);
+            const int v2 = min(max(ovb, mid), e);
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
@@ -447,12 +497,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 acc_add<KAHAN>(ehdb, chdb, bs.h);
                 acc_add<KAHAN>(ehpc, chpc, bs.q);
             }
-            for (int part = 0; part < 2; ++part) {
-                const int fb = part ? mid : b, fe = part ? e : m1;
+            for (int part = 0; part < 4; ++part) {
+                // far [v1, m1), [mid, v2); very far [b, v1), [v2, e)
+                const int fb = part == 0 ? v1 : part == 1 ? mid : part == 2 ? b : v2;
+                const int fe = part == 0 ? m1 : part == 1 ? v2 : part == 2 ? v1 : e;
                 for (int pb = fb; pb < fe; pb += kBlock) {
                     const int pe = min(pb + kBlock, fe);
                     Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                    pair_block_far(ptc, pb, pe, me, q, sc, bs);
+                    if (part < 2)
+                        pair_block_far(ptc, pb, pe, me, q, sc, bs);
+                    else
+                        pair_block_vfar(ptc, pb, pe, me, q, sc, bs);
                     acc_add<KAHAN>(evdw, cvdw, bs.v);
                     acc_add<KAHAN>(eele, cele, bs.e);
                     acc_add<KAHAN>(eele, cele, -bs.ec);
