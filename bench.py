@@ -147,7 +147,7 @@ def run_c2(args, dev, perf, synth):
             "kernel": "mc_kernel", "chunk_ms": chunk_ms, "steps_per_chunk": steps_per_chunk,
             "unit_of_work": "one 100-step chunk of all replicas: the kernel's dispatches for the two "
                             "workgroup groups, run side by side on two streams",
-            "flop_per_pose": f_pose, "fma_issue_peak": perf.FP32_FMA_TFLOPS,
+            "flop_per_pose": f_pose,
             "shape": {"waves": shape_used[0], "slot_waves": shape_used[1], "point_slices": shape_used[2],
                       "replicas_per_group": shape_used[3], "chosen_by": "given" if args.shape else "autotuner"}}
     if traffic:
