@@ -360,12 +360,13 @@ def test_explicit_stream_words(lib, ctx):
     L.check(lib.gd_free_target(ctx, t))
 
 
-@pytest.mark.parametrize("radius", [18.0, 32.0, 48.0, 75.0])
+@pytest.mark.parametrize("radius", [18.0, 24.0, 28.0, 32.0, 48.0, 75.0])
 def test_far_field_detached_ligand(lib, ctx, radius):
     """Ligands translated 18-75 A from the pocket (the reference's positive
     move drift carries them out of the protein): the far field inside the
-    atoms' radius range comes into play; every term stays within tolerance
-    of FP64."""
+    atoms' radius range comes into play, and from about 21 A on the very far
+    field (LJ without a reciprocal), whose boundary the 24 and 28 A cases put
+    inside the protein's points; every term stays within tolerance of FP64."""
     from geauxdock_cs_v2_amd import synth
     cx, _ = synth.load_config("c2")
     n_rep = cx.size["n_rep"]
