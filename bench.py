@@ -21,7 +21,7 @@ Beside it, each in its own section of the JSON line:
   screen    batched screening: 1,024 ligands (lognormal 12-96 atoms, 8
             conformations, 8 temperatures; ~393k replicas) against one
             resident 5,000-point protein in 6 conformations, all jobs in one
-            launch per chunk, on a window of steps; versus plain launches
+            chunk of fused launches, on a window of steps; versus plain launches
             (one job after another, one workgroup per replica) on a sample of
             the ligands;
   serve     the serving loop on an 8,000-point protein in 4 conformations:

@@ -1,9 +1,10 @@
 // Batched form: every replica of several jobs against one resident target in
-// one fused launch per chunk of steps, with replicas packed R per workgroup
-// so their ligand atoms fill the 64-lane waves, the protein points split
-// across `slices` waves, and workgroup shapes picked by a timing autotuner.
-// The chunk sequence (Monte Carlo chunk, exchange round, step advance) can be
-// captured once in a hipGraph and replayed.
+// fused launches per chunk of steps, with replicas packed R per workgroup so
+// their ligand atoms fill the 64-lane waves, the protein points split across
+// `slices` waves, and workgroup shapes picked by a timing autotuner.  The
+// tasks run as two workgroup groups, one launch each per chunk on streams of
+// their own; a group's chunk sequence (Monte Carlo chunk, exchange round,
+// step advance) can be captured once in a hipGraph and replayed.
 #include <algorithm>
 #include <chrono>
 #include <cmath>

@@ -155,7 +155,7 @@ int gd_write_replicas(gd_ctx *ctx, int job_id, const gd_ReplicaMC *in);
 int gd_read_records(gd_ctx *ctx, int job_id, gd_ReplicaMC *out, int *next_entry, int cap);
 
 /* ------------------------------------------------------------------ */
-/* batched form: several jobs in one persistent fused launch per chunk */
+/* batched form: all replicas of several jobs in fused launches per chunk */
 /* ------------------------------------------------------------------ */
 typedef struct gd_batch_opts {
     int steps;             /* MC steps per job */
