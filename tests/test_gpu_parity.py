@@ -445,3 +445,77 @@ def test_docking_driver_known_answer(lib, tmp_path):
     assert np.all(np.abs(np.array([float(x) for x in f[2:]]) - KAT_1A07C1) <= 1.5e-4), out[1]
     dump = (tmp_path / "out_1a07C1.txt").read_text().splitlines()
     assert sum(l.startswith("replica, ") for l in dump) == 51
+
+
+def test_docking_driver_serve_mode(lib, ctx, tmp_path):
+    """gd_dock --serve: the CSV job list as a request stream (the role of the
+    reference's MPI job farm, src/main_mpi_noschedule/server.cpp): three job
+    lines of the 1a07C job (1, 4 and 8 temperatures, 100-200 steps) served
+    together.  Each output line and the last job's final-state dump equal the
+    same requests served in this process (a served result depends on the
+    request alone, not on the process or the mix), and the final states'
+    energies match FP64 at the same poses."""
+    import os
+    import subprocess
+    from conftest import ASTEX
+    from geauxdock_cs_v2_amd import records
+    from geauxdock_cs_v2_amd.lib import PARAM_DIR, PKG_DIR
+    exe = os.path.join(PKG_DIR, "gd_dock")
+    assert os.path.exists(exe), "gd_dock is built by __graft_entry__.build()"
+
+    def line(n_tmp, t_hi, t_lo, n_dump):
+        cols = ["magicnumber_a4a6b25c", "0", "1a07C", os.path.join(ASTEX, "1a07C-1.pdb"), "1a07C1",
+                os.path.join(ASTEX, "1a07C1.sdf"), os.path.join(ASTEX, "1a07C1-0.8.ff"),
+                os.path.join(PARAM_DIR, "08ff_opt"), os.path.join(PARAM_DIR, "paras"),
+                os.path.join(PARAM_DIR, "08_nor_a"), os.path.join(PARAM_DIR, "08_nor_b"),
+                str(n_tmp), str(t_hi), str(t_lo), "0.02", "0.08", str(n_dump)] + ["-2"] * 40
+        return ",".join(cols)
+
+    lines = [line(1, 0.044, 0.036, 1), line(4, 2.0, 0.3, 2), line(8, 2.0, 0.3, 1)]
+    jobs = tmp_path / "jobs.csv"
+    jobs.write_text("header\n" + "\n".join(lines) + "\n")
+    say("case: gd_dock --serve on three 1a07C job lines")
+    r = subprocess.run([exe, str(jobs), "--out", str(tmp_path), "--serve"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.splitlines()
+    assert out[0] == "rep step vdw ele pmf psp hdb hpc kde lhm dst total"
+    assert len(out) == 1 + len(lines), out
+
+    # the same requests served here
+    cxs = [L.load_job_line(l).stage() for l in lines]
+    t = C.c_int()
+    L.check(lib.gd_upload_target(ctx, C.byref(cxs[0].view), C.byref(t)))
+    srv = C.c_void_p()
+    L.check(lib.gd_server_create(ctx, t.value, C.byref(D.ServerOpts(1 << 34, 256, 100, 100)), C.byref(srv)))
+    ids = []
+    for cx in cxs:
+        i = C.c_int()
+        L.check(lib.gd_server_submit(srv, C.byref(cx.view), cx.view.mcpara.steps_total, 2024, C.byref(i)))
+        ids.append(i.value)
+    L.check(lib.gd_server_drain(srv))
+    reps = []
+    for cx, i in zip(cxs, ids):
+        res = D.RequestResult()
+        a = (D.ReplicaMC * cx.size["n_rep"])()
+        L.check(lib.gd_server_poll(srv, i, C.byref(res), a))
+        assert res.status == 0
+        reps.append(a)
+    lib.gd_server_destroy(srv)
+    L.check(lib.gd_free_target(ctx, t.value))
+
+    for k, a in enumerate(reps):
+        assert out[1 + k] == records.csv_line(0, a[0].step, a[0].energy), (k, out[1 + k])
+    rows = [l for l in (tmp_path / "out_1a07C1.txt").read_text().splitlines() if l.startswith("   ")]
+    last = reps[-1]
+    assert len(rows) == len(last)
+    for rr, row in enumerate(rows):
+        want = "   %5d, %5d, %3d, " % (rr, last[rr].step, last[rr].idx_tmp) + "".join(
+            "%8.4f, " % x for x in list(last[rr].movematrix)[:6] + list(last[rr].energy)[:10])
+        assert row == want, (rr, row, want)
+    c = cxs[-1]
+    n = c.size["n_rep"]
+    mm = np.array([list(x.movematrix)[:6] for x in last], np.float32)
+    e64 = O.energy(c.view, np.arange(n, dtype=np.int32), mm, "f64")
+    eg = np.array([list(x.energy)[:10] for x in last])
+    assert rel_err(eg, e64).max() <= REL_TOL
