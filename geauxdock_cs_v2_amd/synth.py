@@ -218,12 +218,13 @@ def make_protein(rng, n_points, n_conf, sigma, radius, spacing=2.8, cavity=8.0):
 
 
 def generate(outdir, lig_id, seed, n_atoms, n_lig_conf, lig_sigma, n_points, n_prt_conf, prt_sigma,
-             prt_radius, n_kde, mcs_rows, mcs_cols, prt_spacing=2.8, n_psp=48, protein=None):
+             prt_radius, n_kde, mcs_rows, mcs_cols, prt_spacing=2.8, n_psp=48, protein=None, lig_radius=6.0):
     """Write <lig_id>.sdf/.ff (and the protein .pdb unless `protein` gives an
-    existing one); returns the paths."""
+    existing one); returns the paths.  Ligand atoms fill a ball of
+    `lig_radius` around the pocket (large ligands need more than 6 A)."""
     rng = np.random.default_rng(seed)
     os.makedirs(outdir, exist_ok=True)
-    confs, types, q = make_ligand(rng, n_atoms, n_lig_conf, lig_sigma)
+    confs, types, q = make_ligand(rng, n_atoms, n_lig_conf, lig_sigma, radius=lig_radius)
     sdf = os.path.join(outdir, f"{lig_id}.sdf")
     write_ligand(sdf, lig_id, confs, types, q)
     if protein is None:

@@ -519,3 +519,54 @@ def test_docking_driver_serve_mode(lib, ctx, tmp_path):
     e64 = O.energy(c.view, np.arange(n, dtype=np.int32), mm, "f64")
     eg = np.array([list(x.energy)[:10] for x in last])
     assert rel_err(eg, e64).max() <= REL_TOL
+
+
+def test_largest_ligand(lib, ctx, tmp_path):
+    """A ligand of 512 atoms (the reference's MAXLIG, src/common/size.h:31) in
+    2 conformations against a 1,500-point protein in 2 conformations, 4
+    temperatures: its atoms span 8 waves, so replicas take 8 slot waves in
+    every form.  Pose energies against FP64, then plain and batched steps
+    against the oracle, step by step."""
+    from geauxdock_cs_v2_amd import synth
+    d = str(tmp_path)
+    g = synth.generate(d, "big512", seed=31, n_atoms=512, n_lig_conf=2, lig_sigma=0.4, n_points=1500,
+                       n_prt_conf=2, prt_sigma=0.4, prt_radius=20.0, n_kde=400, mcs_rows=8, mcs_cols=12,
+                       lig_radius=9.0)
+    raw = L.load_job(g["pdb"], g["sdf"], g["ff"], "big512", n_tmp=4, temp_high=2.0, temp_low=0.3)
+    cx = raw.stage()
+    n = cx.size["n_rep"]
+    assert cx.size["lig_natom"] == 512 and n == 16
+    rng = np.random.default_rng(5)
+    idx = np.arange(n, dtype=np.int32)
+    move = np.concatenate([rng.normal(0, 0.6, (n, 3)), rng.uniform(-math.pi, math.pi, (n, 3))], 1).astype(np.float32)
+    say("case: 512-atom ligand, pose energies")
+    seed = 12
+    t, j = upload(lib, ctx, cx, seed)
+    eg = gpu_energy(lib, ctx, j, idx, move)
+    e64 = O.energy(cx.view, idx, move, "f64")
+    err = rel_err(eg, e64).max(0)
+    say("512 atoms max rel err vs FP64:", np.array2string(err, precision=2, max_line_width=200))
+    assert np.all(np.isfinite(eg)) and err.max() <= REL_TOL, err
+    flips = 0
+    jb = C.c_int()  # a second job of the same target for the batched form
+    L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), seed, C.byref(jb)))
+    for form, job in (("plain", j), ("batched", jb.value)):
+        for step in range(0, 6):
+            say(f"case: 512-atom ligand, {form} step {step}")
+            before = read_reps(lib, ctx, job, n)
+            if form == "plain":
+                L.check(lib.gd_montecarlo(ctx, job, step, 1))
+            else:
+                _batch(lib, ctx, [job], 1)
+            g_ = read_reps(lib, ctx, job, n)
+            o = (D.ReplicaMC * n)()
+            C.memmove(o, before, C.sizeof(o))
+            O.montecarlo(cx.view, o, step, 1, seed)
+            flips += compare_step(lib, cx, seed, step, before, g_, o)
+    L.check(lib.gd_free_job(ctx, jb.value))
+    free(lib, ctx, t, j)
+    # every flip was checked above to be a near-tie (accept uniform within the
+    # energy tolerance of the threshold); the oracle's FP32 totals over 512
+    # atoms widen that window, so a few of the 192 decisions may fall in it
+    say(f"512 atoms: {flips} near-tie flips in {2 * 6 * n} decisions")
+    assert flips <= 12
