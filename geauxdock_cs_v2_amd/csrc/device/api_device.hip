@@ -58,7 +58,7 @@ Shape plain_shape(int natom)
     return s;
 }
 
-int check_stream_range(const JobRes &j, long s1, long s_end)
+int check_stream_range(const JobRes &j, long s_end)
 {
     if (j.kstream_steps > 0 && s_end - 1 > j.kstream_steps)
         return fail(GD_ERR_ARG, "the job's explicit stream words end at step " + std::to_string(j.kstream_steps) +
@@ -640,7 +640,7 @@ int gd_montecarlo(gd_ctx *ctx, int job_id, int s1, int s2max)
     JobRes *j = find_job(ctx, job_id);
     if (!j) return fail(GD_ERR_STATE, "unknown job");
     if (s1 < 0 || s2max <= 0) return fail(GD_ERR_ARG, "bad step range");
-    if (int rc = check_stream_range(*j, s1, (long)s1 + s2max)) return rc;
+    if (int rc = check_stream_range(*j, (long)s1 + s2max)) return rc;
     if (int rc = ensure_records(ctx, *j, std::max(s2max, GD_STEPS_PER_DUMP))) return rc;
     const TargetRes &t = ctx->targets.at(j->target);
     const Shape sh = plain_shape(j->natom);

@@ -393,7 +393,7 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
     for (int i = 0; i < n_jobs; ++i) {
         s1_of[i] = js[i]->steps_done;
         round_of[i] = js[i]->exchange_rounds;
-        if (int rc = check_stream_range(*js[i], s1_of[i], (long)s1_of[i] + opts->steps)) return rc;
+        if (int rc = check_stream_range(*js[i], (long)s1_of[i] + opts->steps)) return rc;
     }
     std::vector<Group> gr = make_groups(js, shapes, gd_ctx::kGroups);
     GroupsIdle idle{ctx};

@@ -101,8 +101,9 @@ struct gd_ctx {
 namespace gd {
 hipError_t upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s);
 int ensure_records(gd_ctx *ctx, JobRes &j, int cap);
-// fails unless the job's explicit stream words (if any) cover steps s1 .. s_end - 1
-int check_stream_range(const JobRes &j, long s1, long s_end);
+// fails unless the job's explicit stream words (if any) cover every step
+// before s_end (steps from 1 on draw words; step 0 draws none)
+int check_stream_range(const JobRes &j, long s_end);
 int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
               int waves, int max_rep, bool compensated, bool records);
 // creates the context's group streams on first use

@@ -142,7 +142,7 @@ __device__ __forceinline__ void lj_term(float d2, float rinv, const PairParams &
 // electrostatic sum cancels by up to 1e6 over a pose, so its terms are added
 // four at a time with compensation by the caller.
 template <bool HPC, bool HDB>
-__device__ __forceinline__ void pair_terms(float d2, float rinv, float ele, float hw, const PairParams &q,
+__device__ __forceinline__ void pair_terms(float d2, float rinv, float hw, const PairParams &q,
                                            const DevScalars &sc, Sums &s)
 {
     const float d = d2 * rinv;
@@ -194,7 +194,7 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_
         float ge = 0.0f; // the group's electrostatics
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            pair_terms<HPC, HDB>(d2[j], rinv[j], P[j].w, HPC ? pa[2 * (p + j)] : 0.0f, q, sc, s);
+            pair_terms<HPC, HDB>(d2[j], rinv[j], HPC ? pa[2 * (p + j)] : 0.0f, q, sc, s);
             ge = fmaf(P[j].w, rinv[j], ge);
         }
         // core corrections where some lane of the wave has a core pair in the group
@@ -210,7 +210,7 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_
         const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float rinv = __builtin_amdgcn_rsqf(d2);
-        pair_terms<HPC, HDB>(d2, rinv, P4.w, HPC ? pa[2 * p] : 0.0f, q, sc, s);
+        pair_terms<HPC, HDB>(d2, rinv, HPC ? pa[2 * p] : 0.0f, q, sc, s);
         kahan_add(s.e, s.ec, fmaf(P4.w, rinv, ele_core_delta(d2, rinv, P4.w, sc)));
     }
 }
