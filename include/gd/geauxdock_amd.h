@@ -40,13 +40,18 @@ enum {
     GD_ERR_BUSY = -6     /* serving: request not finished yet */
 };
 
+/* message of the calling thread's last failure (the reference prints and
+ * exits instead, e.g. src/frontend/frontend.cpp:381-402) */
 const char *gd_last_error(void);
+/* library version string (no counterpart in the reference) */
 const char *gd_version(void);
 
 /* ------------------------------------------------------------------ */
 /* random streams                                                      */
 /* ------------------------------------------------------------------ */
-/* Philox4x32-10 block (Salmon et al., SC'11): out = philox(ctr, key). */
+/* Philox4x32-10 block (Salmon et al., SC'11; the reference vendors
+ * Random123, src/module/Random123, whose kat_vectors:27-29 pin it):
+ * out = philox(ctr, key). */
 void gd_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 /* The injected stream: 31-bit integer k in [0, 2^31-1] for
@@ -88,7 +93,9 @@ void gd_complex_free(gd_complex_handle *h);
  * 9: offsetof(gd_Raw, n_tmp)); lets bindings check their mirrors */
 size_t gd_abi_layout(int which);
 
-/* loader pieces for tests */
+/* Kabsch RMSD after optimal superposition, the conformer filter's measure
+ * (u3b of src/frontend/rmsd.f:16-255, called at src/frontend/load.cpp:360);
+ * exported for tests */
 double gd_superposed_rmsd(const float *ax, const float *ay, const float *az, const float *bx,
                           const float *by, const float *bz, int n);
 
@@ -97,15 +104,19 @@ double gd_superposed_rmsd(const float *ax, const float *ay, const float *az, con
 /* ------------------------------------------------------------------ */
 typedef struct gd_ctx gd_ctx;
 
+/* Docker::Docker / ~Docker device setup and teardown
+ * (src/backend_gpu/backend_gpu.cu:123-167) */
 int gd_create(int device, gd_ctx **out);
 void gd_destroy(gd_ctx *ctx);
-/* HIP device properties of the context's GPU */
+/* HIP device properties of the context's GPU (no counterpart in the reference) */
 int gd_device_info(gd_ctx *ctx, char *name, int name_len, int *cu_count, size_t *hbm_bytes);
 
 /* Protein conformations, EnePara and Psp of a complex, kept resident on the
  * device for any number of jobs (Docker H2D of Complex, backend_gpu.cu:83,
  * split so the target is uploaded once). */
 int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id);
+/* releases a target's device tables (the device copy's cudaFree in ~Docker,
+ * backend_gpu.cu:151-158) */
 int gd_free_target(gd_ctx *ctx, int target_id);
 
 /* Ligand conformations, KDE, MCS, temperatures, replicas and move scales of
@@ -121,8 +132,10 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
  * draw; step 0 draws nothing).  Launches that need steps beyond n_steps
  * fail with GD_ERR_ARG.  k_words null returns the job to Philox. */
 int gd_set_stream(gd_ctx *ctx, int job_id, uint64_t seed, const uint32_t *k_words, int n_steps);
+/* releases a job's device tables (as gd_free_target) */
 int gd_free_job(gd_ctx *ctx, int job_id);
-/* device bytes a job of these sizes takes (admission by HBM footprint) */
+/* device bytes a job of these sizes takes (admission by HBM footprint; no
+ * counterpart in the reference, whose Complex has static capacities) */
 size_t gd_job_footprint(int lig_natom, int n_lig, int kde_npoint, int mcs_entries, int mcs_nrow,
                         int n_tmp, int n_rep, int record_cap);
 
@@ -197,9 +210,11 @@ void gd_server_destroy(gd_server *srv);
 /* queue a request: the complex's job tables, steps, seed; returns an id */
 int gd_server_submit(gd_server *srv, const gd_Complex *cx, int steps, uint64_t seed, int *req_id);
 int gd_server_poll(gd_server *srv, int req_id, gd_request_result *res, gd_ReplicaMC *final_replicas);
+/* cancel a queued or running request; it finishes with GD_ERR_STATE */
 int gd_server_cancel(gd_server *srv, int req_id);
 /* run the serving loop until every queued request is finished */
 int gd_server_drain(gd_server *srv);
+/* the server clock (ms since gd_server_create) of gd_request_result's times */
 double gd_server_clock_ms(gd_server *srv);
 
 #ifdef __cplusplus
