@@ -570,3 +570,42 @@ def test_largest_ligand(lib, ctx, tmp_path):
     # atoms widen that window, so a few of the 192 decisions may fall in it
     say(f"512 atoms: {flips} near-tie flips in {2 * 6 * n} decisions")
     assert flips <= 12
+
+
+def test_c1_full_length_trajectory(lib, ctx):
+    """The C1 workload at its full length (48 replicas, 2,000 steps, exchange
+    every 100 steps, as Dock runs it, src/backend_gpu/backend_gpu.cu:91-107):
+    every plain step against the oracle from the same state (compare_step),
+    every exchange round's labels against the oracle's.  The positive-only
+    move drift carries the ligands out of the protein within the run, so the
+    Monte Carlo path passes through the near, far and very far fields."""
+    from geauxdock_cs_v2_amd import synth
+    cx, _ = synth.load_config("c1")
+    n = cx.size["n_rep"]
+    seed = 21
+    t, j = upload(lib, ctx, cx, seed)
+    flips = 0
+    rnd = 0
+    say("case: c1 full length, 2000 plain steps + exchange every 100")
+    for step in range(0, 2000):
+        before = read_reps(lib, ctx, j, n)
+        L.check(lib.gd_montecarlo(ctx, j, step, 1))
+        g = read_reps(lib, ctx, j, n)
+        o = (D.ReplicaMC * n)()
+        C.memmove(o, before, C.sizeof(o))
+        O.montecarlo(cx.view, o, step, 1, seed)
+        flips += compare_step(lib, cx, seed, step, before, g, o)
+        if (step + 1) % 100 == 0:
+            acc = C.c_int()
+            L.check(lib.gd_exchange(ctx, j, rnd, C.byref(acc)))
+            want = O.exchange(cx.view, g, rnd, seed)
+            after = read_reps(lib, ctx, j, n)
+            assert acc.value == want, (step, rnd)
+            assert [after[r].idx_tmp for r in range(n)] == [g[r].idx_tmp for r in range(n)], (step, rnd)
+            rnd += 1
+            if rnd % 5 == 0:
+                dist = np.array([np.linalg.norm(after[r].movematrix[:3]) for r in range(n)])
+                say(f"  step {step + 1}: {flips} near-tie flips, ligand offsets {dist.min():.1f}-{dist.max():.1f} A")
+    free(lib, ctx, t, j)
+    say(f"c1 full length: {flips} near-tie flips in {2000 * n} decisions")
+    assert flips <= 20
