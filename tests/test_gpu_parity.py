@@ -609,3 +609,44 @@ def test_c1_full_length_trajectory(lib, ctx):
     free(lib, ctx, t, j)
     say(f"c1 full length: {flips} near-tie flips in {2000 * n} decisions")
     assert flips <= 20
+
+
+@pytest.mark.parametrize("which,chunks", [("c1", 20), ("kat11", 4)])
+def test_batched_chunks_vs_oracle(lib, ctx, kat_complex11, which, chunks):
+    """The batched form as the benchmark runs it (autotuned shape, workgroup
+    groups, hipGraph-replayed 100-step chunks with the exchange round inside)
+    chunk by chunk against the oracle run from the same state: C1 over its
+    full 2,000 steps (one protein conformation: one group), and 1a07C with 11
+    protein conformations and 4 temperatures (2,244 replicas: two groups, each
+    exchanging its own ladders) over 400 steps.  A replica whose 100 decisions
+    all agree ends the chunk with the bit-identical move, step and temperature
+    label and energies within tolerance of FP64; a near-tie decision sends a
+    replica (and its exchange partners) on another trajectory, which only a
+    few may do."""
+    from geauxdock_cs_v2_amd import synth
+    cx = kat_complex11[1] if which == "kat11" else synth.load_config("c1")[0]
+    n = cx.size["n_rep"]
+    seed = 33
+    t, j = upload(lib, ctx, cx, seed)
+    diverged = 0
+    say(f"case: {which} batched, {chunks} chunks of 100 steps with exchange, against the oracle")
+    for k in range(chunks):
+        before = read_reps(lib, ctx, j, n)
+        _batch(lib, ctx, [j], 100, exchange=100, graph=1, comp=1)
+        g = read_reps(lib, ctx, j, n)
+        o = (D.ReplicaMC * n)()
+        C.memmove(o, before, C.sizeof(o))
+        O.montecarlo(cx.view, o, 100 * k, 100, seed)
+        O.exchange(cx.view, o, k, seed)
+        same = [list(g[r].movematrix) == list(o[r].movematrix) and g[r].step == o[r].step
+                and g[r].idx_tmp == o[r].idx_tmp for r in range(n)]
+        diverged += n - sum(same)
+        idx = np.array([r for r in range(n) if same[r]], np.int32)
+        mm = np.array([list(g[r].movematrix)[:6] for r in idx], np.float32)
+        e64 = O.energy(cx.view, idx, mm, "f64")
+        eg = np.array([list(g[r].energy)[:10] for r in idx])
+        assert rel_err(eg, e64).max() <= REL_TOL, k
+        # the next chunk starts from the GPU's state (the oracle is resynchronised)
+    free(lib, ctx, t, j)
+    say(f"{which} batched: {diverged} replica-chunks of {chunks * n} left the oracle's trajectory")
+    assert diverged <= max(12, chunks * n // 100)
