@@ -207,7 +207,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     // conformations (in the float coordinates the kernel reads): within a
     // segment points go nearest first, so the points beyond any radius are a
     // suffix of the segment
-    std::vector<double> rmin(np, 1e300), rmax(np, 0.0);
+    std::vector<double> rmin(np, 1e300);
     for (int c = 0; c < nconf; ++c)
         for (int i = 0; i < np; ++i) {
             const size_t k = (size_t)c * np + i;
@@ -215,7 +215,6 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
                          z = local(cx->prt_z[k], origin[2]);
             const double r = std::sqrt(x * x + y * y + z * z);
             rmin[i] = std::min(rmin[i], r);
-            rmax[i] = std::max(rmax[i], r);
         }
     std::vector<int> ord(np);
     std::iota(ord.begin(), ord.end(), 0);
@@ -224,21 +223,6 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         if (flags[x] != flags[y]) return flags[x] > flags[y];
         return rmin[x] < rmin[y];
     });
-    std::vector<float2> pa(np);
-    for (int i = 0; i < np; ++i) {
-        const int s = ord[i];
-        float2 v;
-        v.x = 0.5f * hpcw[s]; // the factored contact polynomial leaves a factor 1/2 to the weight
-        std::memcpy(&v.y, &cx->prt_t[s], 4);
-        pa[i] = v;
-    }
-    std::vector<float4> pt((size_t)nconf * np);
-    for (int c = 0; c < nconf; ++c)
-        for (int i = 0; i < np; ++i) {
-            const size_t s = (size_t)c * np + ord[i];
-            pt[(size_t)c * np + i] = make_float4(local(cx->prt_x[s], origin[0]), local(cx->prt_y[s], origin[1]),
-                                                 local(cx->prt_z[s], origin[2]), cx->prt_ele[ord[i]]);
-        }
     std::vector<int4> seg;
     for (int i = 0; i < np;) {
         const int t = cx->prt_t[ord[i]], f = flags[ord[i]];
@@ -325,17 +309,63 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             vfar_d += 0.05;
         }
     }
-    std::vector<int> ladder(seg.size() * kFarRungs), ladder_in(seg.size() * kFarRungs);
-    for (size_t g = 0; g < seg.size(); ++g)
-        for (int k = 0; k < kFarRungs; ++k) {
-            const double rk = kFarR0 + k * kFarDr;
-            int i = seg[g].y;
-            while (i < seg[g].z && rmin[ord[i]] < rk) ++i;
-            ladder[g * kFarRungs + k] = i;
-            int m = seg[g].y;
-            while (m < seg[g].z && rmax[ord[m]] <= rk) ++m;
-            ladder_in[g * kFarRungs + k] = m;
+    // One copy of the points and ladders per ladder centre C_k = k kCenterDr
+    // (1,1,1)/sqrt(3): within each segment the points go by their smallest
+    // distance from C_k over the conformations, and the ladders count
+    // distances from C_k (copy 0, C_0 = the pocket centre, is the order
+    // above).  The kernel takes the copy whose centre is nearest the wave's
+    // atoms; any copy gives valid bounds.
+    const size_t nseg = seg.size();
+    std::vector<float2> pa((size_t)kCenters * np);
+    std::vector<float4> pt((size_t)kCenters * nconf * np);
+    std::vector<int> ladder((size_t)kCenters * nseg * kFarRungs), ladder_in((size_t)kCenters * nseg * kFarRungs);
+    std::vector<double> dmin(np), dmax(np);
+    std::vector<int> ordk(np);
+    for (int kc = 0; kc < kCenters; ++kc) {
+        const double cc = (double)(float)(kc * kCenterDr * 0.57735026918962576f); // each coordinate of C_k, as the kernel rounds it
+        for (int i = 0; i < np; ++i) {
+            dmin[i] = 1e300;
+            dmax[i] = 0.0;
         }
+        for (int c = 0; c < nconf; ++c)
+            for (int i = 0; i < np; ++i) {
+                const size_t k = (size_t)c * np + i;
+                const double x = (double)local(cx->prt_x[k], origin[0]) - cc, y = (double)local(cx->prt_y[k], origin[1]) - cc,
+                             z = (double)local(cx->prt_z[k], origin[2]) - cc;
+                const double r = std::sqrt(x * x + y * y + z * z);
+                dmin[i] = std::min(dmin[i], r);
+                dmax[i] = std::max(dmax[i], r);
+            }
+        ordk = ord;
+        if (kc > 0)
+            for (const int4 &g : seg)
+                std::stable_sort(ordk.begin() + g.y, ordk.begin() + g.z,
+                                 [&](int x, int y) { return dmin[x] < dmin[y]; });
+        for (int i = 0; i < np; ++i) {
+            const int s = ordk[i];
+            float2 v;
+            v.x = 0.5f * hpcw[s]; // the factored contact polynomial leaves a factor 1/2 to the weight
+            std::memcpy(&v.y, &cx->prt_t[s], 4);
+            pa[(size_t)kc * np + i] = v;
+        }
+        for (int c = 0; c < nconf; ++c)
+            for (int i = 0; i < np; ++i) {
+                const size_t s = (size_t)c * np + ordk[i];
+                pt[((size_t)kc * nconf + c) * np + i] =
+                    make_float4(local(cx->prt_x[s], origin[0]), local(cx->prt_y[s], origin[1]),
+                                local(cx->prt_z[s], origin[2]), cx->prt_ele[ordk[i]]);
+            }
+        for (size_t g = 0; g < nseg; ++g)
+            for (int k = 0; k < kFarRungs; ++k) {
+                const double rk = kFarR0 + k * kFarDr;
+                int i = seg[g].y;
+                while (i < seg[g].z && dmin[ordk[i]] < rk) ++i;
+                ladder[((size_t)kc * nseg + g) * kFarRungs + k] = i;
+                int m = seg[g].y;
+                while (m < seg[g].z && dmax[ordk[m]] <= rk) ++m;
+                ladder_in[((size_t)kc * nseg + g) * kFarRungs + k] = m;
+            }
+    }
 
     const int id = ctx->next_id++;
     TargetRes &t = ctx->targets[id];

@@ -2,19 +2,20 @@
 // uploads and launches them.
 //
 // HBM layout (one resident target, any number of jobs):
-//   target  pt   float4[nconf][npad]  {x, y, z, ele} of every effective point
-//                                     in a frame centred on the pocket centre,
-//                                     re-sorted by (point type, hydrophobic
-//                                     flag) so a run of points shares one set
-//                                     of type-pair parameters and terms
-//           pa   float2[npad]         {hpc weight = (cdc ? hpp : 0) / 2, point type}
-//           seg  int4[nseg]           {point type, begin, end, hydrophobic flag};
-//                                     within a segment points go by their
-//                                     smallest radius over the conformations
-//           ladder int[nseg][kFarRungs] first point of each segment at or
-//                                     beyond each radius rung (far field)
-//           ladder_in int[nseg][kFarRungs] end of each segment's leading points
-//                                     whose largest radius is within the rung
+//   target  pt   float4[kCenters][nconf][npad]  {x, y, z, ele} of every
+//                                     effective point in a frame centred on the
+//                                     pocket centre, re-sorted by (point type,
+//                                     hydrophobic flag) so a run of points (a
+//                                     segment) shares one set of type-pair
+//                                     parameters and terms; copy k orders each
+//                                     segment by distance from ladder centre k
+//           pa   float2[kCenters][npad] {hpc weight = (cdc ? hpp : 0) / 2, point type}
+//           seg  int4[nseg]           {point type, begin, end, hydrophobic flag}
+//           ladder int[kCenters][nseg][kFarRungs] first point of each segment at
+//                                     or beyond each distance rung from the
+//                                     centre (far field)
+//           ladder_in int[kCenters][nseg][kFarRungs] end of each segment's leading
+//                                     points whose largest distance is within the rung
 //           pp   float[31][24][PP_N]  type-pair parameters, see PairParam
 //   job     lig3 float4[nlig][natom]  {x3, y3, z3, c3} type-sorted atoms
 //           lig2 float4[nlig][natom]  {x2, y2, z2, 0} file-order atoms (MCS)
@@ -68,18 +69,28 @@ struct DevScalars {
     float a_para[GD_MAXWEI], b_para[GD_MAXWEI], w[GD_MAXWEI];
 };
 
+// Ladder centres: the far-field ladders are built around kCenters points of
+// the (1,1,1) axis through the pocket centre, kCenterDr apart; the
+// reference's moves add non-negative increments to all three translation
+// components, so a drifting ligand stays near that axis.
+constexpr int kCenters = 16;
+constexpr float kCenterDr = 4.0f;
+
 // Far-field radius ladder: rung k stands for radius kFarR0 + k * kFarDr.
 constexpr int kFarRungs = 128;
 constexpr float kFarR0 = 8.0f;
 constexpr float kFarDr = 0.5f;
 
 struct DevTarget {
-    const float4 *pt;  // [nconf][npad] {x, y, z, ele}, pocket-centred frame
-    const float2 *pa;  // [npad] {hydrophobic weight / 2, point type (int bits)}
+    // pt, pa and the ladders come in kCenters copies: copy k orders each
+    // segment's points by their distance from ladder centre k (copy 0: the
+    // pocket centre); the segments themselves are the same in every copy
+    const float4 *pt;  // [kCenters][nconf][npad] {x, y, z, ele}, pocket-centred frame
+    const float2 *pa;  // [kCenters][npad] {hydrophobic weight / 2, point type (int bits)}
     const float *pp;   // [31][24][PP_N] type-pair parameters
     const int4 *seg;   // [nseg] {point type, begin, end, hydrophobic flag}
-    const int *ladder; // [nseg][kFarRungs] first point of the segment at radius >= rung
-    const int *ladder_in; // [nseg][kFarRungs] end of the segment's leading points with largest radius <= rung
+    const int *ladder; // [kCenters][nseg][kFarRungs] first point of the segment at distance >= rung from the centre
+    const int *ladder_in; // [kCenters][nseg][kFarRungs] end of the segment's leading points with largest distance <= rung
     float far_d;       // distance beyond which only LJ and 1/d electrostatics vary
     float vfar_d;      // distance beyond which 1 / (1 + e) of the soft-core LJ is 1 - e + e^2 to float rounding
     int npoint, npad, nseg, nconf;

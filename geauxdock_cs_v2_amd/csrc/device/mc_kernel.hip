@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     const int tl = J.lt[a];
     const float lc = L3.w;
     const int prt = task.prt;
-    const float4 *__restrict__ ptc = tg.pt + (size_t)prt * tg.npad;
+    const float4 *__restrict__ ptc0 = tg.pt + (size_t)prt * tg.npad; // ladder centre 0: the pocket-centre order
     const int np = tg.npoint;
     const int lo = (int)(((long long)np * slice) / S);
     const int hi = (int)(((long long)np * (slice + 1)) / S);
@@ -439,7 +439,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // rung kf of a segment's ladder on points are at least far_d beyond
         // every atom, and up to ladder_in's rung ki they are at least far_d
         // inside every atom (a ligand that has left the pocket)
-        const float r2 = fmaf(me.x, me.x, fmaf(me.y, me.y, me.z * me.z));
+        // ladder centre: the point of the drift axis (1,1,1)/sqrt(3) nearest
+        // the middle of the wave's atoms (their projections' mid-range)
+        const float ax = (me.x + me.y + me.z) * 0.57735026918962576f;
+        const int kc = __builtin_amdgcn_readfirstlane(
+            (int)fminf(fmaxf(rintf(0.5f * (wave_max(ax) + wave_min(ax)) * (1.0f / kCenterDr)), 0.0f),
+                       (float)(kCenters - 1)));
+        const float cc = (float)kc * kCenterDr * 0.57735026918962576f;
+        const float4 *__restrict__ ptc = tg.pt + ((size_t)kc * tg.nconf + prt) * tg.npad;
+        const float2 *__restrict__ pak = tg.pa + (size_t)kc * tg.npad;
+        const float ux = me.x - cc, uy = me.y - cc, uz = me.z - cc;
+        const float r2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
         const float rho_out = sqrtf(wave_max(r2)), rho_in = sqrtf(wave_min(r2));
         const int kf = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho_out + tg.far_d - kFarR0) * (1.0f / kFarDr))));
         const int ki = __builtin_amdgcn_readfirstlane(
@@ -450,8 +460,8 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             (int)fminf(fmaxf(ceilf((rho_out + tg.vfar_d - kFarR0) * (1.0f / kFarDr)), 0.0f), (float)kFarRungs));
         const int kvi = __builtin_amdgcn_readfirstlane(
             (int)fmaxf(fminf(floorf((rho_in - tg.vfar_d - kFarR0) * (1.0f / kFarDr)), (float)(kFarRungs - 1)), -1.0f));
-        const cint *ladder = (const cint *)tg.ladder;
-        const cint *ladder_in = (const cint *)tg.ladder_in;
+        const cint *ladder = (const cint *)(tg.ladder + (size_t)kc * tg.nseg * kFarRungs);
+        const cint *ladder_in = (const cint *)(tg.ladder_in + (size_t)kc * tg.nseg * kFarRungs);
         for (int si = 0; si < tg.nseg; ++si) {
             const int4 sg = tg.seg[si];
             const int b = max(sg.y, lo), e = min(sg.z, hi);
@@ -481,14 +491,14 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (sg.w) { // segment flag: hydrophobic-contact points
                     if (hdb)
-                        pair_block<true, true>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                        pair_block<true, true>(ptc, pak, pb, pe, me, q, sc, bs);
                     else
-                        pair_block<true, false>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                        pair_block<true, false>(ptc, pak, pb, pe, me, q, sc, bs);
                 } else {
                     if (hdb)
-                        pair_block<false, true>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                        pair_block<false, true>(ptc, pak, pb, pe, me, q, sc, bs);
                     else
-                        pair_block<false, false>(ptc, tg.pa, pb, pe, me, q, sc, bs);
+                        pair_block<false, false>(ptc, pak, pb, pe, me, q, sc, bs);
                 }
                 acc_add<KAHAN>(evdw, cvdw, bs.v);
                 acc_add<KAHAN>(eele, cele, bs.e);
@@ -523,7 +533,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             for (int k = 0; k < J.npsp; ++k) {
                 const int p = J.psp_pt[k];
                 if (p < lo || p >= hi) continue; // the slice that owns the point adds it
-                const float4 P4 = ptc[p];
+                const float4 P4 = ptc0[p];
                 const int tp = __float_as_int(tg.pa[p].y);
                 const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
