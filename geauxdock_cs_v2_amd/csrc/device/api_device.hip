@@ -309,20 +309,36 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             vfar_d += 0.05;
         }
     }
-    // One copy of the points and ladders per ladder centre C_k = k kCenterDr
+    // One copy of the points and ladders per ladder centre C_k = k center_dr
     // (1,1,1)/sqrt(3): within each segment the points go by their smallest
     // distance from C_k over the conformations, and the ladders count
     // distances from C_k (copy 0, C_0 = the pocket centre, is the order
     // above).  The kernel takes the copy whose centre is nearest the wave's
     // atoms; any copy gives valid bounds.
     const size_t nseg = seg.size();
+    // centre spacing: the last centre lies a ligand's extent (12 A) beyond
+    // the protein's radius plus the very far distance, never closer than 4 A
+    float center_dr;
+    {
+        double prot_r = 0.0;
+        for (int c = 0; c < nconf; ++c)
+            for (int i = 0; i < np; ++i) {
+                const size_t k = (size_t)c * np + i;
+                const double x = local(cx->prt_x[k], origin[0]), y = local(cx->prt_y[k], origin[1]),
+                             z = local(cx->prt_z[k], origin[2]);
+                prot_r = std::max(prot_r, std::sqrt(x * x + y * y + z * z));
+            }
+        const double reach = prot_r + std::min(vfar_d, far_d + 64.0) + 12.0;
+        center_dr = (float)std::max(4.0, reach / (kCenters - 1));
+    }
     std::vector<float2> pa((size_t)kCenters * np);
     std::vector<float4> pt((size_t)kCenters * nconf * np);
     std::vector<int> ladder((size_t)kCenters * nseg * kFarRungs), ladder_in((size_t)kCenters * nseg * kFarRungs);
     std::vector<double> dmin(np), dmax(np);
     std::vector<int> ordk(np);
     for (int kc = 0; kc < kCenters; ++kc) {
-        const double cc = (double)(float)(kc * kCenterDr * 0.57735026918962576f); // each coordinate of C_k, as the kernel rounds it
+        const float ccf = (float)kc * center_dr * 0.57735026918962576f; // each coordinate of C_k, rounded as the kernel does
+        const double cc = (double)ccf;
         for (int i = 0; i < np; ++i) {
             dmin[i] = 1e300;
             dmax[i] = 0.0;
@@ -379,6 +395,8 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.ladder_in = t.ladder_in.as<int>();
     t.dt.far_d = (float)far_d;
     t.dt.vfar_d = (float)std::min(vfar_d, 1e30);
+    t.dt.center_dr = center_dr;
+    t.dt.inv_center_dr = 1.0f / center_dr;
     t.dt.pt = t.pt.as<float4>();
     t.dt.pa = t.pa.as<float2>();
     t.dt.pp = t.pp.as<float>();

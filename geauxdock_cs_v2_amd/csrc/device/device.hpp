@@ -70,11 +70,10 @@ struct DevScalars {
 };
 
 // Ladder centres: the far-field ladders are built around kCenters points of
-// the (1,1,1) axis through the pocket centre, kCenterDr apart; the
-// reference's moves add non-negative increments to all three translation
-// components, so a drifting ligand stays near that axis.
+// the (1,1,1) axis through the pocket centre, DevTarget::center_dr apart;
+// the reference's moves add non-negative increments to all three
+// translation components, so a drifting ligand stays near that axis.
 constexpr int kCenters = 16;
-constexpr float kCenterDr = 4.0f;
 
 // Far-field radius ladder: rung k stands for radius kFarR0 + k * kFarDr.
 constexpr int kFarRungs = 128;
@@ -93,6 +92,10 @@ struct DevTarget {
     const int *ladder_in; // [kCenters][nseg][kFarRungs] end of the segment's leading points with largest distance <= rung
     float far_d;       // distance beyond which only LJ and 1/d electrostatics vary
     float vfar_d;      // distance beyond which 1 / (1 + e) of the soft-core LJ is 1 - e + e^2 to float rounding
+    // spacing of the ladder centres: they reach the protein's radius plus
+    // the very far distance and a ligand's extent; a wave beyond the last
+    // takes centre 0, whose inner ladder then puts every point far
+    float center_dr, inv_center_dr;
     int npoint, npad, nseg, nconf;
 };
 

@@ -440,12 +440,13 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // every atom, and up to ladder_in's rung ki they are at least far_d
         // inside every atom (a ligand that has left the pocket)
         // ladder centre: the point of the drift axis (1,1,1)/sqrt(3) nearest
-        // the middle of the wave's atoms (their projections' mid-range)
+        // the middle of the wave's atoms (their projections' mid-range);
+        // beyond the last centre, centre 0 (the protein is then behind the
+        // atoms, and the inner ladder bounds every point)
         const float ax = (me.x + me.y + me.z) * 0.57735026918962576f;
-        const int kc = __builtin_amdgcn_readfirstlane(
-            (int)fminf(fmaxf(rintf(0.5f * (wave_max(ax) + wave_min(ax)) * (1.0f / kCenterDr)), 0.0f),
-                       (float)(kCenters - 1)));
-        const float cc = (float)kc * kCenterDr * 0.57735026918962576f;
+        const float kpos = rintf(0.5f * (wave_max(ax) + wave_min(ax)) * tg.inv_center_dr);
+        const int kc = __builtin_amdgcn_readfirstlane(kpos > (float)(kCenters - 1) ? 0 : (int)fmaxf(kpos, 0.0f));
+        const float cc = (float)kc * tg.center_dr * 0.57735026918962576f;
         const float4 *__restrict__ ptc = tg.pt + ((size_t)kc * tg.nconf + prt) * tg.npad;
         const float2 *__restrict__ pak = tg.pa + (size_t)kc * tg.npad;
         const float ux = me.x - cc, uy = me.y - cc, uz = me.z - cc;
