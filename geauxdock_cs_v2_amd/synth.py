@@ -201,10 +201,14 @@ def make_ligand(rng, n_atoms, n_conf, sigma, spacing=1.2, radius=6.0):
     q = rng.normal(0.0, 0.3, n_atoms)
     q -= q.mean()
     confs = [nat]
+    tries = 0
     while len(confs) < n_conf:
         c = nat + rng.normal(0.0, sigma, nat.shape)
         if _kabsch_rmsd(c, nat) > 0.15:
             confs.append(c)
+        tries += 1
+        if tries > 100000:  # e.g. one atom: every conformer superposes exactly
+            raise RuntimeError("cannot draw distinct ligand conformers")
     return np.array(confs), types, q
 
 

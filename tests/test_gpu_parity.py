@@ -650,3 +650,49 @@ def test_batched_chunks_vs_oracle(lib, ctx, kat_complex11, which, chunks):
     free(lib, ctx, t, j)
     say(f"{which} batched: {diverged} replica-chunks of {chunks * n} left the oracle's trajectory")
     assert diverged <= max(12, chunks * n // 100)
+
+
+@pytest.mark.parametrize("n_atoms,n_points", [(2, 7), (3, 33)])
+def test_tiny_inputs(lib, ctx, tmp_path, n_atoms, n_points):
+    """Ragged small inputs: 2- and 3-atom ligands against 7- and 33-point
+    proteins (fewer points than point slices, one-point segments, groups of
+    four with a remainder, atom types without KDE points), 4 temperatures:
+    pose energies against FP64, then plain and batched steps against the
+    oracle, step by step."""
+    from geauxdock_cs_v2_amd import synth
+    g = synth.generate(str(tmp_path), f"t{n_atoms}", seed=3, n_atoms=n_atoms, n_lig_conf=2, lig_sigma=0.4,
+                       n_points=n_points, n_prt_conf=2, prt_sigma=0.4, prt_radius=12.0, n_kde=4, mcs_rows=2,
+                       mcs_cols=12)
+    raw = L.load_job(g["pdb"], g["sdf"], g["ff"], f"t{n_atoms}", n_tmp=4, temp_high=2.0, temp_low=0.3)
+    cx = raw.stage()
+    n = cx.size["n_rep"]
+    assert cx.size["prt_npoint"] == n_points and cx.size["lig_natom"] == n_atoms
+    rng = np.random.default_rng(9)
+    idx = np.arange(n, dtype=np.int32)
+    move = np.concatenate([rng.normal(0, 2.0, (n, 3)), rng.uniform(-math.pi, math.pi, (n, 3))], 1).astype(np.float32)
+    say(f"case: tiny inputs {n_atoms} atoms x {n_points} points, pose energies")
+    seed = 14
+    t, j = upload(lib, ctx, cx, seed)
+    eg = gpu_energy(lib, ctx, j, idx, move)
+    e64 = O.energy(cx.view, idx, move, "f64")
+    err = rel_err(eg, e64).max(0)
+    assert np.all(np.isfinite(eg)) and err.max() <= REL_TOL, err
+    jb = C.c_int()
+    L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), seed, C.byref(jb)))
+    flips = 0
+    for form, job in (("plain", j), ("batched", jb.value)):
+        for step in range(0, 6):
+            say(f"case: tiny inputs {n_atoms} x {n_points}, {form} step {step}")
+            before = read_reps(lib, ctx, job, n)
+            if form == "plain":
+                L.check(lib.gd_montecarlo(ctx, job, step, 1))
+            else:
+                _batch(lib, ctx, [job], 1)
+            g_ = read_reps(lib, ctx, job, n)
+            o = (D.ReplicaMC * n)()
+            C.memmove(o, before, C.sizeof(o))
+            O.montecarlo(cx.view, o, step, 1, seed)
+            flips += compare_step(lib, cx, seed, step, before, g_, o)
+    L.check(lib.gd_free_job(ctx, jb.value))
+    free(lib, ctx, t, j)
+    assert flips <= 2
