@@ -378,7 +378,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": c2["wall"] * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": c2["workload"], "replicas": c2["n_rep"],
-                   "form": "batched (packed replicas, autotuned shape, hipGraph chunks)"},
+                   "form": "batched (packed replicas, autotuned shape, two workgroup groups on two streams, hipGraph chunks)"},
         "plain": plain,
         "speedup_vs_plain": value / plain["value"] if (value and plain["value"]) else None,
         "roofline": c2["roofline"],
