@@ -319,6 +319,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     // centre spacing: the last centre lies a ligand's extent (12 A) beyond
     // the protein's radius plus the very far distance, never closer than 4 A
     float center_dr;
+    double t_prot_r = 0.0;
     {
         double prot_r = 0.0;
         for (int c = 0; c < nconf; ++c)
@@ -330,6 +331,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             }
         const double reach = prot_r + std::min(vfar_d, far_d + 64.0) + 12.0;
         center_dr = (float)std::max(4.0, reach / (kCenters - 1));
+        t_prot_r = prot_r;
     }
     std::vector<float2> pa((size_t)kCenters * np);
     std::vector<float4> pt((size_t)kCenters * nconf * np);
@@ -397,6 +399,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.vfar_d = (float)std::min(vfar_d, 1e30);
     t.dt.center_dr = center_dr;
     t.dt.inv_center_dr = 1.0f / center_dr;
+    t.dt.prot_r = (float)t_prot_r + 1e-3f; // rounded up: a rung at or beyond it holds every point
     t.dt.pt = t.pt.as<float4>();
     t.dt.pa = t.pa.as<float2>();
     t.dt.pp = t.pp.as<float>();

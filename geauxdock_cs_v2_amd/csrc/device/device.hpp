@@ -96,6 +96,7 @@ struct DevTarget {
     // the very far distance and a ligand's extent; a wave beyond the last
     // takes centre 0, whose inner ladder then puts every point far
     float center_dr, inv_center_dr;
+    float prot_r; // largest distance of a point from the pocket centre over the conformations
     int npoint, npad, nseg, nconf;
 };
 

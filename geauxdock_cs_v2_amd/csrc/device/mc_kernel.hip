@@ -461,8 +461,33 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             (int)fminf(fmaxf(ceilf((rho_out + tg.vfar_d - kFarR0) * (1.0f / kFarDr)), 0.0f), (float)kFarRungs));
         const int kvi = __builtin_amdgcn_readfirstlane(
             (int)fmaxf(fminf(floorf((rho_in - tg.vfar_d - kFarR0) * (1.0f / kFarDr)), (float)(kFarRungs - 1)), -1.0f));
+        // A wave whose atoms are all beyond the protein by vfar_d: from centre
+        // 0, rung kvi of the inner ladder then holds every point of every
+        // segment, so all pairs are very far and a lean loop walks the slice's
+        // points without the per-segment splits or the PSP points (none of
+        // which is within pmf0): the same arithmetic in the same order
+        const bool lean = kc == 0 && kvi >= 0 && kFarR0 + (float)kvi * kFarDr >= tg.prot_r;
         const cint *ladder = (const cint *)(tg.ladder + (size_t)kc * tg.nseg * kFarRungs);
         const cint *ladder_in = (const cint *)(tg.ladder_in + (size_t)kc * tg.nseg * kFarRungs);
+        if (lean) {
+            for (int si = 0; si < tg.nseg; ++si) {
+                const int4 sg = tg.seg[si];
+                const int b = max(sg.y, lo), e = min(sg.z, hi);
+                if (b >= e) continue;
+                const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
+                const float4 pA = *(const float4 *)P;
+                const PairParams q{pA.x, pA.y, pA.w, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                for (int pb = b; pb < e; pb += kBlock) {
+                    const int pe = min(pb + kBlock, e);
+                    Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    pair_block_vfar(ptc, pb, pe, me, q, sc, bs);
+                    acc_add<KAHAN>(evdw, cvdw, bs.v);
+                    acc_add<KAHAN>(eele, cele, bs.e);
+                    acc_add<KAHAN>(eele, cele, -bs.ec);
+                    acc_add<KAHAN>(epmf, cpmf, bs.p);
+                }
+            }
+        } else {
         for (int si = 0; si < tg.nseg; ++si) {
             const int4 sg = tg.seg[si];
             const int b = max(sg.y, lo), e = min(sg.z, hi);
@@ -544,6 +569,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             }
             acc_add<KAHAN>(epsp, cpsp, ss);
         }
+        } // general path
 
         // KDE over the atom type's run of the type-sorted table
         // (energy_kde_v1.cpp:1-77): slice s takes points kr.x + s, kr.x + s + S, ...
