@@ -228,10 +228,10 @@ int join_groups(gd_ctx *ctx, hipStream_t s, size_t n, Event *ev)
 
 } // namespace
 
-// Autotuner: time `trial` steps of every candidate workgroup shape on the
-// job's own data (state saved and restored around the trials), launched in
-// workgroup groups as the batched runs are, and keep the fastest; results
-// are cached per (N_L, N_P, n_rep, n_prt).
+// Autotuner: time a short run of consecutive launches of every candidate
+// workgroup shape on the job's own data (state saved and restored around the
+// trials), launched in workgroup groups as the batched runs are, and keep the
+// fastest; results are cached per (N_L, N_P, n_rep, n_prt).
 int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool compensated, Shape &out)
 {
     const auto key = std::make_tuple(j.natom, t.npoint, j.n_rep, j.n_prt);
@@ -266,11 +266,17 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
     Event e0, e1, ef, ej[gd_ctx::kGroups];
     for (Event *e : {&e0, &e1, &ef}) GD_HIP(hipEventCreate(&e->e));
     for (Event &e : ej) GD_HIP(hipEventCreateWithFlags(&e.e, hipEventDisableTiming));
-    // two timed steps per candidate; one, untimed warm-up dropped, once a
-    // step holds more than ~1e11 pair evaluations (a few 100 ms)
+    // Each candidate is timed as the real chunk sequence runs: three
+    // consecutive 2-step launches per group, alternating over the group
+    // streams, so that one group's tail overlaps the other's next launch (a
+    // single launch per group times the tails the sequence hides, and ranks
+    // C2's best shapes within noise of each other; three launches rank them
+    // as full 10,000-step runs do).  Once a step holds more than ~1e11 pair
+    // evaluations (a few 100 ms) one 1-step launch per group, no warm-up.
     const double pairs = (double)j.n_rep * j.natom * t.npoint;
     const bool large = pairs > 1e11;
     const int trial = large ? 1 : 2;
+    const int trial_chunks = large ? 1 : 3;
     float best_ms = 1e30f;
     Shape best = cands[0];
     std::vector<JobRes *> one{&j};
@@ -283,13 +289,15 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
             group_tasks(g, one, shapes, s1_of, trial, g.tasks);
             GD_HIP(upload(g.tasks_dev, g.tasks.data(), sizeof(DevTask) * g.tasks.size(), s));
         }
-        // warm the code object and caches once, then time
+        // warm the code object and caches with one launch per group, then time
         for (int rep = large ? 1 : 0; rep < 2; ++rep) {
             GD_HIP(hipEventRecord(e0.e, s));
             if (int rc = fork_groups(ctx, s, gr.size(), ef.e)) return rc;
-            for (size_t g = 0; g < gr.size(); ++g)
-                GD_HIP(launch_mc(t.dt, t.sc, j.self.as<DevJob>(), gr[g].tasks_dev.as<DevTask>(), (int)gr[g].tasks.size(),
-                                 c.waves, c.nrep, compensated, false, ctx->group_stream[g], nullptr));
+            for (int k = 0; k < (rep ? trial_chunks : 1); ++k)
+                for (size_t g = 0; g < gr.size(); ++g)
+                    GD_HIP(launch_mc(t.dt, t.sc, j.self.as<DevJob>(), gr[g].tasks_dev.as<DevTask>(),
+                                     (int)gr[g].tasks.size(), c.waves, c.nrep, compensated, false,
+                                     ctx->group_stream[g], nullptr));
             if (int rc = join_groups(ctx, s, gr.size(), ej)) return rc;
             GD_HIP(hipEventRecord(e1.e, s));
             GD_HIP(hipMemcpyAsync(j.rep.p, backup.p, j.rep.bytes, hipMemcpyDeviceToDevice, s));
