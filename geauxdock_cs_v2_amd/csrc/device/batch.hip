@@ -260,9 +260,12 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
     if (cands.empty()) return fail(GD_ERR_ARG, "ligand too large for a 16-wave workgroup");
     if (int rc = ensure_group_streams(ctx)) return rc;
     hipStream_t s = ctx->stream;
-    DevBuf backup;
+    // the trials change the replica states and the acceptance counters
+    DevBuf backup, backup_log;
     GD_HIP(backup.alloc(j.rep.bytes));
+    GD_HIP(backup_log.alloc(j.mclog.bytes));
     GD_HIP(hipMemcpyAsync(backup.p, j.rep.p, j.rep.bytes, hipMemcpyDeviceToDevice, s));
+    GD_HIP(hipMemcpyAsync(backup_log.p, j.mclog.p, j.mclog.bytes, hipMemcpyDeviceToDevice, s));
     Event e0, e1, ef, ej[gd_ctx::kGroups];
     for (Event *e : {&e0, &e1, &ef}) GD_HIP(hipEventCreate(&e->e));
     for (Event &e : ej) GD_HIP(hipEventCreateWithFlags(&e.e, hipEventDisableTiming));
@@ -301,6 +304,7 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
             if (int rc = join_groups(ctx, s, gr.size(), ej)) return rc;
             GD_HIP(hipEventRecord(e1.e, s));
             GD_HIP(hipMemcpyAsync(j.rep.p, backup.p, j.rep.bytes, hipMemcpyDeviceToDevice, s));
+            GD_HIP(hipMemcpyAsync(j.mclog.p, backup_log.p, j.mclog.bytes, hipMemcpyDeviceToDevice, s));
             GD_HIP(hipEventSynchronize(e1.e));
         }
         float ms = 0.f;

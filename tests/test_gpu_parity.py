@@ -6,12 +6,13 @@ accept and exchange decisions identical except where the accept uniform sits
 within the energy tolerance of the threshold."""
 import ctypes as C
 import math
+import os
 import sys
 
 import numpy as np
 import pytest
 
-from conftest import KAT_1A07C1, REL_TOL, TERM_FLOOR, rel_err
+from conftest import ASTEX, KAT_1A07C1, REL_TOL, TERM_FLOOR, rel_err
 from geauxdock_cs_v2_amd import ctypes_defs as D
 from geauxdock_cs_v2_amd import lib as L
 from oracle import bind as O
@@ -414,6 +415,28 @@ def test_mclog_counters(lib, ctx, kat_complex11):
     assert np.array_equal(mc, moves)
     assert np.array_equal(ex, swaps)
     assert swaps.sum() > 0
+
+
+def test_mclog_after_autotuned_batch(lib, ctx):
+    """The autotuner's trial launches leave the job as they found it: after an
+    autotuned batched chunk the McLog accept counters equal the chunk's
+    records (minus the step-0 entry).  Three temperatures give this job a
+    shape key of its own, so the autotuner runs its trials here."""
+    raw = L.load_job(os.path.join(ASTEX, "1a07C-1.pdb"), os.path.join(ASTEX, "1a07C1.sdf"),
+                     os.path.join(ASTEX, "1a07C1-0.8.ff"), "1a07C1", n_tmp=3, temp_high=2.0, temp_low=0.3)
+    cx = raw.stage()
+    n = cx.size["n_rep"]
+    t, j = upload(lib, ctx, cx, 43)
+    say("case: McLog counters after an autotuned batched chunk, 1a07C1 x 3 temperatures, 50 steps")
+    _batch(lib, ctx, [j], 50, records=1)
+    rec = (D.ReplicaMC * (n * 100))()
+    ne = np.zeros(n, np.int32)
+    L.check(lib.gd_read_records(ctx, j, rec, ne.ctypes.data_as(C.POINTER(C.c_int)), 100))
+    mc = np.zeros(n, np.int32)
+    L.check(lib.gd_read_mclog(ctx, j, mc.ctypes.data_as(C.POINTER(C.c_int)), None))
+    free(lib, ctx, t, j)
+    assert (ne > 1).any()
+    assert np.array_equal(mc, ne - 1)
 
 
 def test_docking_driver_known_answer(lib, tmp_path):
