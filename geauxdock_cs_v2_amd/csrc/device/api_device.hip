@@ -120,6 +120,27 @@ JobRes *find_job(gd_ctx *ctx, int id)
     return it == ctx->jobs.end() ? nullptr : &it->second;
 }
 
+// The replica layout the kernels index by (Loader::set_replica,
+// src/frontend/frontend.cpp:267-299): replica r = n_tmp n_lig p + n_lig t + l
+// belongs to ligand conformation l and protein conformation p, and every
+// (l, p) ladder holds each temperature exactly once (the exchange kernel
+// finds a ladder's replicas through their temperature labels).
+int check_replica_layout(const gd_ReplicaMC *rep, int n_lig, int n_prt, int n_tmp)
+{
+    std::vector<int> seen(n_tmp);
+    for (int p = 0; p < n_prt; ++p)
+        for (int l = 0; l < n_lig; ++l) {
+            std::fill(seen.begin(), seen.end(), 0);
+            for (int t = 0; t < n_tmp; ++t) {
+                const gd_ReplicaMC &r = rep[(size_t)n_tmp * n_lig * p + (size_t)n_lig * t + l];
+                if (r.idx_prt != p || r.idx_lig != l || r.idx_tmp < 0 || r.idx_tmp >= n_tmp)
+                    return fail(GD_ERR_ARG, "replica index fields out of range or out of place");
+                if (seen[r.idx_tmp]++) return fail(GD_ERR_ARG, "ladder temperatures are not a permutation");
+            }
+        }
+    return GD_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -387,6 +408,18 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
 
     const int id = ctx->next_id++;
     TargetRes &t = ctx->targets[id];
+    // a failed upload leaves no target behind
+    struct Drop {
+        gd_ctx *ctx;
+        int id;
+        bool keep = false;
+        ~Drop()
+        {
+            if (keep) return;
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->targets.erase(id);
+        }
+    } drop{ctx, id};
     GD_HIP(upload(t.pt, pt.data(), sizeof(float4) * pt.size(), ctx->stream));
     GD_HIP(upload(t.pa, pa.data(), sizeof(float2) * pa.size(), ctx->stream));
     GD_HIP(upload(t.pp, pp.data(), sizeof(float) * pp.size(), ctx->stream));
@@ -447,6 +480,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.prt_t.assign(cx->prt_t, cx->prt_t + np);
     t.prt_c.assign(cx->prt_c, cx->prt_c + np);
     GD_HIP(hipStreamSynchronize(ctx->stream));
+    drop.keep = true;
     *target_id = id;
     return GD_OK;
 }
@@ -486,6 +520,8 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     if (z.prt_npoint != tit->second.npoint) return fail(GD_ERR_ARG, "job and target point counts differ");
     if (z.n_rep != z.n_lig * z.n_prt * z.n_tmp) return fail(GD_ERR_ARG, "n_rep != n_lig * n_prt * n_tmp");
     const int na = z.lig_natom;
+    if (!cx->replica) return fail(GD_ERR_ARG, "null replica table");
+    if (int rc = check_replica_layout(cx->replica, z.n_lig, z.n_prt, z.n_tmp)) return rc;
 
     std::vector<float4> lig3((size_t)z.n_lig * na), lig2((size_t)z.n_lig * na);
     for (size_t i = 0; i < lig3.size(); ++i) {
@@ -555,6 +591,18 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
 
     const int id = ctx->next_id++;
     JobRes &j = ctx->jobs[id];
+    // a failed upload leaves no job behind (the device buffers free with it)
+    struct Drop {
+        gd_ctx *ctx;
+        int id;
+        bool keep = false;
+        ~Drop()
+        {
+            if (keep) return;
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->jobs.erase(id);
+        }
+    } drop{ctx, id};
     j.target = target_id;
     hipStream_t s = ctx->stream;
     GD_HIP(upload(j.lig3, lig3.data(), sizeof(float4) * lig3.size(), s));
@@ -617,18 +665,12 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     j.nrow = z.mcs_nrow;
     j.seed = seed;
     j.rep_prt.resize(z.n_rep);
-    for (int r = 0; r < z.n_rep; ++r) {
-        j.rep_prt[r] = cx->replica[r].idx_prt;
-        if (j.rep_prt[r] < 0 || j.rep_prt[r] >= z.n_prt || cx->replica[r].idx_lig < 0 ||
-            cx->replica[r].idx_lig >= z.n_lig || cx->replica[r].idx_tmp < 0 || cx->replica[r].idx_tmp >= z.n_tmp) {
-            ctx->jobs.erase(id);
-            return fail(GD_ERR_ARG, "replica index fields out of range");
-        }
-    }
+    for (int r = 0; r < z.n_rep; ++r) j.rep_prt[r] = cx->replica[r].idx_prt;
     j.bytes = j.lig3.bytes + j.lig2.bytes + j.lt.bytes + j.krange.bytes + j.kde.bytes + j.mrow.bytes + j.ment.bytes +
               j.mtcc.bytes + j.mbeta.bytes + j.psp_pt.bytes + j.psp_row.bytes + j.rep.bytes + j.next_entry.bytes +
               j.self.bytes + j.mclog.bytes;
     GD_HIP(hipStreamSynchronize(s));
+    drop.keep = true;
     *job_id = id;
     return GD_OK;
 }
@@ -760,21 +802,7 @@ int gd_write_replicas(gd_ctx *ctx, int job_id, const gd_ReplicaMC *in)
     if (int rc = check_ctx(ctx)) return rc;
     JobRes *j = find_job(ctx, job_id);
     if (!j || !in) return fail(GD_ERR_ARG, "unknown job or null input");
-    for (int r = 0; r < j->n_rep; ++r)
-        if (in[r].idx_prt != j->rep_prt[r] || in[r].idx_tmp < 0 || in[r].idx_tmp >= j->n_tmp || in[r].idx_lig < 0 ||
-            in[r].idx_lig >= j->n_lig)
-            return fail(GD_ERR_ARG, "replica index fields out of range or protein conformation changed");
-    // every (ligand conformation, protein conformation) ladder must hold each
-    // temperature exactly once (the exchange kernel indexes by it)
-    for (int p = 0; p < j->n_prt; ++p)
-        for (int l = 0; l < j->n_lig; ++l) {
-            std::vector<int> seen(j->n_tmp, 0);
-            for (int t = 0; t < j->n_tmp; ++t) {
-                const gd_ReplicaMC &r = in[j->n_tmp * j->n_lig * p + j->n_lig * t + l];
-                if (r.idx_lig != l || seen[r.idx_tmp]++)
-                    return fail(GD_ERR_ARG, "ladder temperatures are not a permutation");
-            }
-        }
+    if (int rc = check_replica_layout(in, j->n_lig, j->n_prt, j->n_tmp)) return rc;
     GD_HIP(hipMemcpyAsync(j->rep.p, in, sizeof(gd_ReplicaMC) * j->n_rep, hipMemcpyHostToDevice, ctx->stream));
     GD_HIP(hipStreamSynchronize(ctx->stream));
     return GD_OK;

@@ -417,6 +417,46 @@ def test_mclog_counters(lib, ctx, kat_complex11):
     assert swaps.sum() > 0
 
 
+def test_bad_replica_layouts_are_refused(lib, ctx, kat_complex11):
+    """Replica tables whose ladders do not hold each temperature once, or whose
+    replicas sit out of their (ligand, protein conformation) place, are refused
+    at upload and at write, before any launch reads them."""
+    cx = kat_complex11[1]
+    n = cx.size["n_rep"]
+    n_lig, n_tmp = cx.size["n_lig"], cx.size["n_tmp"]
+    t = C.c_int()
+    L.check(lib.gd_upload_target(ctx, C.byref(cx.view), C.byref(t)))
+
+    def broken(kind):
+        reps = cx.replicas()
+        if kind == "duplicate temperature":
+            reps[n_lig].idx_tmp = reps[0].idx_tmp  # ladder (0, 0): temperatures 0, 0, 2, 3
+        elif kind == "protein conformation out of place":
+            reps[0].idx_prt = 1
+        else:  # ligand conformation out of place
+            reps[0].idx_lig = 1 % n_lig if n_lig > 1 else 0
+            reps[0].idx_tmp = n_tmp  # out of range when there is one conformation
+        return reps
+
+    for kind in ("duplicate temperature", "protein conformation out of place", "ligand conformation out of place"):
+        say(f"case: upload and write refuse a replica table with a {kind}")
+        reps = broken(kind)
+        view = D.Complex.from_buffer_copy(cx.view)
+        view.replica = C.cast(reps, C.POINTER(D.ReplicaMC))
+        j = C.c_int(-1)
+        assert lib.gd_upload_job(ctx, t, C.byref(view), 3, C.byref(j)) == L.GD_ERR_ARG
+        jg = C.c_int()
+        L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), 3, C.byref(jg)))
+        assert lib.gd_write_replicas(ctx, jg, reps) == L.GD_ERR_ARG
+        # the job still runs from its own table
+        L.check(lib.gd_montecarlo(ctx, jg, 0, 2))
+        g = read_reps(lib, ctx, jg, n)
+        assert sorted(r.idx_tmp for r in g[:: n_lig][:n_tmp]) == list(range(n_tmp))
+        L.check(lib.gd_free_job(ctx, jg))
+    # a refused upload left no job behind: the target frees
+    L.check(lib.gd_free_target(ctx, t))
+
+
 def test_mclog_after_autotuned_batch(lib, ctx):
     """The autotuner's trial launches leave the job as they found it: after an
     autotuned batched chunk the McLog accept counters equal the chunk's
