@@ -548,6 +548,7 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     const float *org = tit->second.origin;
     for (int k = 0; k < z.kde_npoint; ++k)
         kde[k] = make_float4(local(cx->kde_x[k], org[0]), local(cx->kde_y[k], org[1]), local(cx->kde_z[k], org[2]), 0.0f);
+    if (cx->mcs_rowptr[0] != 0) return fail(GD_ERR_ARG, "MCS row pointers must start at 0");
     const int nent = cx->mcs_rowptr[z.mcs_nrow];
     std::vector<int> mrow(cx->mcs_rowptr, cx->mcs_rowptr + z.mcs_nrow + 1);
     std::vector<float4> ment(std::max(nent, 1));

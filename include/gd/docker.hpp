@@ -44,9 +44,20 @@ class Docker {
         if (!complex) throw std::runtime_error("Docker::run: no complex");
         const gd_ComplexSize &z = complex->size;
         const gd_McPara &mc = complex->mcpara;
-        int target = -1, job = -1;
-        check(gd_upload_target(ctx_, complex, &target));
-        check(gd_upload_job(ctx_, target, complex, seed, &job));
+        if (mc.steps_per_dump <= 0 || mc.steps_total < 0) throw std::runtime_error("Docker::run: bad step counts");
+        // the job and target are freed on every way out of run()
+        struct Held {
+            gd_ctx *ctx;
+            int target = -1, job = -1;
+            ~Held()
+            {
+                if (job >= 0) gd_free_job(ctx, job);
+                if (target >= 0) gd_free_target(ctx, target);
+            }
+        } held{ctx_};
+        check(gd_upload_target(ctx_, complex, &held.target));
+        check(gd_upload_job(ctx_, held.target, complex, seed, &held.job));
+        const int job = held.job;
         const int cap = mc.steps_per_dump > record_cap ? mc.steps_per_dump : record_cap;
         record.assign((size_t)z.n_rep * cap, gd_ReplicaMC());
         next_entry.assign(z.n_rep, 0);
@@ -60,8 +71,6 @@ class Docker {
         replica.assign(z.n_rep, gd_ReplicaMC());
         check(gd_read_replicas(ctx_, job, replica.data()));
         record_cap = cap;
-        check(gd_free_job(ctx_, job));
-        check(gd_free_target(ctx_, target));
     }
 
   private:
