@@ -121,7 +121,11 @@ int gd_free_target(gd_ctx *ctx, int target_id);
 
 /* Ligand conformations, KDE, MCS, temperatures, replicas and move scales of
  * a complex, docked against a resident target.  `seed` keys the job's
- * injected random stream. */
+ * injected random stream.  The replica table must have the layout
+ * Loader::set_replica builds (src/frontend/frontend.cpp:267-299): replica
+ * n_tmp*n_lig*p + n_lig*t + l of ligand conformation l and protein
+ * conformation p, each (l, p) ladder holding every temperature once;
+ * otherwise GD_ERR_ARG (gd_write_replicas checks the same). */
 int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t seed, int *job_id);
 /* The job's injected stream (replaces the reference's time-seeded rand_r,
  * kernel_cpp_montecarlo_soa.cpp:8-18, 233-234, 376): `seed` re-keys the
