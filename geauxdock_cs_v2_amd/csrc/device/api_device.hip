@@ -307,29 +307,34 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     // X = c p1a / d^9; e falls with d (lj_a, lj_b >= 0).  Beyond vfar_d,
     // e <= 3e-3 for every type pair, so 1 - e + e^2 equals 1 / (1 + e) to
     // within e^3 < 2^-25 and the kernel needs no reciprocal there.
-    double vfar_d = far_d;
-    {
-        const double kEps = 3.0e-3;
+    // Two more tiers for ligands that have left the protein far behind:
+    // beyond lean_e_d, e <= 2^-12.5, so the e^2 term is below 2^-25 and
+    // n (1 - e) holds; beyond lean_soft_d, e <= 2^-25 and the pair is n.
+    // soft_reach(eps): the smallest distance from `from` on at which
+    // e <= eps for every type pair (1e30 when the softening does not fall
+    // with d).
+    auto soft_reach = [&](double eps, double from) {
         const double ljc_ = ep.lj0 != 0.0f ? (double)ep.lj0 : 1.0;
         const double a = ep.lj0 / ljc_, b = ep.lj0 / ljc_ * ep.lj1;
-        if (a < 0.0 || b < 0.0) {
-            vfar_d = 1e30; // softening that does not fall with d: no very far field
-        } else {
-            for (int tp = 0; tp < GD_MAXTP1; ++tp)
-                for (int tl = 0; tl < GD_MAXTP2; ++tl) {
-                    const double p1 = (double)(float)(ljc_ * ep.p1a[tl][tp]);
-                    auto e_at = [&](double d) { return p1 * (a + b * d * d) / std::pow(d, 9); };
-                    if (p1 <= 0.0 || e_at(vfar_d) <= kEps) continue;
-                    double lo = vfar_d, hi = 4096.0;
-                    for (int it = 0; it < 80; ++it) {
-                        const double m = 0.5 * (lo + hi);
-                        (e_at(m) > kEps ? lo : hi) = m;
-                    }
-                    vfar_d = hi;
+        if (a < 0.0 || b < 0.0) return 1e30;
+        double reach = from;
+        for (int tp = 0; tp < GD_MAXTP1; ++tp)
+            for (int tl = 0; tl < GD_MAXTP2; ++tl) {
+                const double p1 = (double)(float)(ljc_ * ep.p1a[tl][tp]);
+                auto e_at = [&](double d) { return p1 * (a + b * d * d) / std::pow(d, 9); };
+                if (p1 <= 0.0 || e_at(reach) <= eps) continue;
+                double lo = reach, hi = 65536.0;
+                for (int it = 0; it < 80; ++it) {
+                    const double m = 0.5 * (lo + hi);
+                    (e_at(m) > eps ? lo : hi) = m;
                 }
-            vfar_d += 0.05;
-        }
-    }
+                reach = hi;
+            }
+        return reach + 0.05;
+    };
+    const double vfar_d = soft_reach(3.0e-3, far_d);
+    const double lean_e_d = vfar_d < 1e30 ? soft_reach(std::ldexp(std::sqrt(0.5), -12), vfar_d) : 1e30;
+    const double lean_soft_d = vfar_d < 1e30 ? soft_reach(std::ldexp(1.0, -25), lean_e_d) : 1e30;
     // One copy of the points and ladders per ladder centre C_k = k center_dr
     // (1,1,1)/sqrt(3): within each segment the points go by their smallest
     // distance from C_k over the conformations, and the ladders count
@@ -430,6 +435,8 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.ladder_in = t.ladder_in.as<int>();
     t.dt.far_d = (float)far_d;
     t.dt.vfar_d = (float)std::min(vfar_d, 1e30);
+    t.dt.lean_e_d = (float)std::min(lean_e_d, 1e30);
+    t.dt.lean_soft_d = (float)std::min(lean_soft_d, 1e30);
     t.dt.center_dr = center_dr;
     t.dt.inv_center_dr = 1.0f / center_dr;
     t.dt.prot_r = (float)t_prot_r + 1e-3f; // rounded up: a rung at or beyond it holds every point

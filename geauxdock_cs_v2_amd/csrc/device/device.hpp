@@ -92,6 +92,8 @@ struct DevTarget {
     const int *ladder_in; // [kCenters][nseg][kFarRungs] end of the segment's leading points with largest distance <= rung
     float far_d;       // distance beyond which only LJ and 1/d electrostatics vary
     float vfar_d;      // distance beyond which 1 / (1 + e) of the soft-core LJ is 1 - e + e^2 to float rounding
+    float lean_e_d;    // distance beyond which it is 1 - e (e <= 2^-12.5)
+    float lean_soft_d; // distance beyond which it is 1 (e <= 2^-25)
     // spacing of the ladder centres: they reach the protein's radius plus
     // the very far distance and a ligand's extent; a wave beyond the last
     // takes centre 0, whose inner ladder then puts every point far

@@ -250,7 +250,10 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int p
 // Very far field: points at least vfar_d from every atom of the wave, where
 // the LJ softening e = X (lj_a + lj_b d2) is at most 3e-3 for every type
 // pair (DevTarget::vfar_d): n / (1 + e) = n (1 - e + e^2) to float rounding,
-// so the pair needs one transcendental (1/d) instead of two.
+// so the pair needs one transcendental (1/d) instead of two.  TIER 1: every
+// pair beyond lean_e_d (e <= 2^-12.5), where n (1 - e) is as close; TIER 2:
+// beyond lean_soft_d (e <= 2^-25), where n alone is.
+template <int TIER>
 __device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, int pb, int pe, const Slot &me,
                                                 const PairParams &q, const DevScalars &sc, Sums &s)
 {
@@ -268,9 +271,16 @@ __device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, int pb, int 
             const float r3 = r2 * rinv;
             const float r6 = r3 * r3;
             const float X = q.p1a * (r6 * r3);
-            const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
             const float n = fmaf(-q.p2a, r6, X);
-            s.v += fmaf(-n, fmaf(-e, e, e), n); // n - n (e - e^2)
+            if (TIER == 0) {
+                const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
+                s.v += fmaf(-n, fmaf(-e, e, e), n); // n - n (e - e^2)
+            } else if (TIER == 1) {
+                const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
+                s.v += fmaf(-n, e, n); // n - n e
+            } else {
+                s.v += n;
+            }
             ge = fmaf(P4.w, rinv, ge);
         }
         kahan_add(s.e, s.ec, ge);
@@ -467,6 +477,11 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // points without the per-segment splits or the PSP points (none of
         // which is within pmf0): the same arithmetic in the same order
         const bool lean = kc == 0 && kvi >= 0 && kFarR0 + (float)kvi * kFarDr >= tg.prot_r;
+        // the points lie within prot_r of centre 0, so on the lean path every
+        // pair is at least rho_in - prot_r apart: beyond lean_e_d and
+        // lean_soft_d the softening takes fewer terms
+        const float gap = rho_in - tg.prot_r;
+        const int tier = __builtin_amdgcn_readfirstlane(gap >= tg.lean_soft_d ? 2 : (gap >= tg.lean_e_d ? 1 : 0));
         const cint *ladder = (const cint *)(tg.ladder + (size_t)kc * tg.nseg * kFarRungs);
         const cint *ladder_in = (const cint *)(tg.ladder_in + (size_t)kc * tg.nseg * kFarRungs);
         if (lean) {
@@ -480,7 +495,12 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 for (int pb = b; pb < e; pb += kBlock) {
                     const int pe = min(pb + kBlock, e);
                     Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                    pair_block_vfar(ptc, pb, pe, me, q, sc, bs);
+                    if (tier == 2)
+                        pair_block_vfar<2>(ptc, pb, pe, me, q, sc, bs);
+                    else if (tier == 1)
+                        pair_block_vfar<1>(ptc, pb, pe, me, q, sc, bs);
+                    else
+                        pair_block_vfar<0>(ptc, pb, pe, me, q, sc, bs);
                     acc_add<KAHAN>(evdw, cvdw, bs.v);
                     acc_add<KAHAN>(eele, cele, bs.e);
                     acc_add<KAHAN>(eele, cele, -bs.ec);
@@ -543,7 +563,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                     if (part < 2)
                         pair_block_far(ptc, pb, pe, me, q, sc, bs);
                     else
-                        pair_block_vfar(ptc, pb, pe, me, q, sc, bs);
+                        pair_block_vfar<0>(ptc, pb, pe, me, q, sc, bs);
                     acc_add<KAHAN>(evdw, cvdw, bs.v);
                     acc_add<KAHAN>(eele, cele, bs.e);
                     acc_add<KAHAN>(eele, cele, -bs.ec);

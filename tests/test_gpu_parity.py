@@ -361,13 +361,15 @@ def test_explicit_stream_words(lib, ctx):
     L.check(lib.gd_free_target(ctx, t))
 
 
-@pytest.mark.parametrize("radius", [18.0, 24.0, 28.0, 32.0, 48.0, 75.0])
+@pytest.mark.parametrize("radius", [18.0, 24.0, 28.0, 32.0, 48.0, 75.0, 160.0, 250.0])
 def test_far_field_detached_ligand(lib, ctx, radius):
-    """Ligands translated 18-75 A from the pocket (the reference's positive
+    """Ligands translated 18-250 A from the pocket (the reference's positive
     move drift carries them out of the protein): the far field inside the
     atoms' radius range comes into play, and from about 21 A on the very far
     field (LJ without a reciprocal), whose boundary the 24 and 28 A cases put
-    inside the protein's points; every term stays within tolerance of FP64."""
+    inside the protein's points; beyond about 32 A from every point the LJ
+    softening loses its e^2 term (75 A) and beyond about 109 A the softening
+    itself (160, 250 A); every term stays within tolerance of FP64."""
     from geauxdock_cs_v2_amd import synth
     cx, _ = synth.load_config("c2")
     n_rep = cx.size["n_rep"]
@@ -674,25 +676,34 @@ def test_c1_full_length_trajectory(lib, ctx):
     assert flips <= 20
 
 
-@pytest.mark.parametrize("which,chunks", [("c1", 20), ("kat11", 4)])
-def test_batched_chunks_vs_oracle(lib, ctx, kat_complex11, which, chunks):
+@pytest.mark.parametrize("which,chunks,offset", [("c1", 20, 0.0), ("kat11", 4, 0.0), ("c1", 3, 70.0),
+                                                 ("c1", 3, 150.0)])
+def test_batched_chunks_vs_oracle(lib, ctx, kat_complex11, which, chunks, offset):
     """The batched form as the benchmark runs it (autotuned shape, workgroup
     groups, hipGraph-replayed 100-step chunks with the exchange round inside)
     chunk by chunk against the oracle run from the same state: C1 over its
     full 2,000 steps (one protein conformation: one group), and 1a07C with 11
     protein conformations and 4 temperatures (2,244 replicas: two groups, each
-    exchanging its own ladders) over 400 steps.  A replica whose 100 decisions
-    all agree ends the chunk with the bit-identical move, step and temperature
-    label and energies within tolerance of FP64; a near-tie decision sends a
-    replica (and its exchange partners) on another trajectory, which only a
-    few may do."""
+    exchanging its own ladders) over 400 steps; C1 also from poses moved 70
+    and 150 A along the drift axis, where the ligands are 44 and 124 A from
+    every protein point (the lean very far path and its two shorter LJ
+    softenings).  A replica whose 100 decisions all agree ends the chunk with
+    the bit-identical move, step and temperature label and energies within
+    tolerance of FP64; a near-tie decision sends a replica (and its exchange
+    partners) on another trajectory, which only a few may do."""
     from geauxdock_cs_v2_amd import synth
     cx = kat_complex11[1] if which == "kat11" else synth.load_config("c1")[0]
     n = cx.size["n_rep"]
     seed = 33
     t, j = upload(lib, ctx, cx, seed)
+    if offset:
+        reps = read_reps(lib, ctx, j, n)
+        for r in range(n):
+            for k in range(3):
+                reps[r].movematrix[k] += offset / math.sqrt(3.0)
+        L.check(lib.gd_write_replicas(ctx, j, reps))
     diverged = 0
-    say(f"case: {which} batched, {chunks} chunks of 100 steps with exchange, against the oracle")
+    say(f"case: {which} batched, {chunks} chunks of 100 steps with exchange, offset {offset} A, against the oracle")
     for k in range(chunks):
         before = read_reps(lib, ctx, j, n)
         _batch(lib, ctx, [j], 100, exchange=100, graph=1, comp=1)
