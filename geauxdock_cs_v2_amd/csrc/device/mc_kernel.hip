@@ -259,6 +259,7 @@ __device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, int pb, int 
 {
     const cfloat *ptc = (const cfloat *)ptc_;
     int p = pb;
+    float v2 = 0.0f; // TIER 2: the attractive part, summed apart
     for (; p + 4 <= pe; p += 4) {
         float ge = 0.0f;
 #pragma unroll
@@ -270,21 +271,27 @@ __device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, int pb, int 
             const float r2 = rinv * rinv;
             const float r3 = r2 * rinv;
             const float r6 = r3 * r3;
-            const float X = q.p1a * (r6 * r3);
-            const float n = fmaf(-q.p2a, r6, X);
             if (TIER == 0) {
+                const float X = q.p1a * (r6 * r3);
+                const float n = fmaf(-q.p2a, r6, X);
                 const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
                 s.v += fmaf(-n, fmaf(-e, e, e), n); // n - n (e - e^2)
             } else if (TIER == 1) {
-                const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
-                s.v += fmaf(-n, e, n); // n - n e
+                const float X = q.p1a * (r6 * r3);
+                const float n = fmaf(-q.p2a, r6, X);
+                s.v = fmaf(n, fmaf(-X, fmaf(sc.lj_b, d2, sc.lj_a), 1.0f), s.v); // n (1 - e)
             } else {
-                s.v += n;
+                // n = X - c p2a r^6 as two sums: at this range the repulsive
+                // part is a small fraction of the attractive one (below 1e-4
+                // for the shipped parameters), so they do not cancel
+                s.v = fmaf(q.p1a, r6 * r3, s.v);
+                v2 = fmaf(q.p2a, r6, v2);
             }
             ge = fmaf(P4.w, rinv, ge);
         }
         kahan_add(s.e, s.ec, ge);
     }
+    if (TIER == 2) s.v -= v2;
     for (; p < pe; ++p) {
         const float4 P4 = ld_point(ptc, p);
         const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
