@@ -402,6 +402,12 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     const int np = tg.npoint;
     const int lo = (int)(((long long)np * slice) / S);
     const int hi = (int)(((long long)np * (slice + 1)) / S);
+    // the segments that overlap this slice's points [lo, hi): the segments
+    // cover the points in order, so they form one run [seg_b, seg_e)
+    int seg_b = 0;
+    while (seg_b < tg.nseg && tg.seg[seg_b].z <= lo) ++seg_b;
+    int seg_e = seg_b;
+    while (seg_e < tg.nseg && tg.seg[seg_e].y < hi) ++seg_e;
 
     for (int s2 = 0; s2 < s2max; ++s2) {
         const int step = s1 + s2;
@@ -492,12 +498,19 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         const cint *ladder = (const cint *)(tg.ladder + (size_t)kc * tg.nseg * kFarRungs);
         const cint *ladder_in = (const cint *)(tg.ladder_in + (size_t)kc * tg.nseg * kFarRungs);
         if (lean) {
-            for (int si = 0; si < tg.nseg; ++si) {
-                const int4 sg = tg.seg[si];
+            // the next segment's bounds and parameters load while this
+            // segment's points run
+            int4 sgn = seg_b < seg_e ? tg.seg[seg_b] : make_int4(0, 0, 0, 0);
+            float4 pAn = *(const float4 *)(tg.pp + ((size_t)sgn.x * GD_MAXTP2 + me.tl) * PP_N);
+            for (int si = seg_b; si < seg_e; ++si) {
+                const int4 sg = sgn;
+                const float4 pA = pAn;
+                if (si + 1 < seg_e) {
+                    sgn = tg.seg[si + 1];
+                    pAn = *(const float4 *)(tg.pp + ((size_t)sgn.x * GD_MAXTP2 + me.tl) * PP_N);
+                }
                 const int b = max(sg.y, lo), e = min(sg.z, hi);
                 if (b >= e) continue;
-                const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
-                const float4 pA = *(const float4 *)P;
                 const PairParams q{pA.x, pA.y, pA.w, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 for (int pb = b; pb < e; pb += kBlock) {
                     const int pe = min(pb + kBlock, e);
@@ -515,7 +528,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 }
             }
         } else {
-        for (int si = 0; si < tg.nseg; ++si) {
+        for (int si = seg_b; si < seg_e; ++si) {
             const int4 sg = tg.seg[si];
             const int b = max(sg.y, lo), e = min(sg.z, hi);
             if (b >= e) continue;
