@@ -253,53 +253,59 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int p
 // so the pair needs one transcendental (1/d) instead of two.  TIER 1: every
 // pair beyond lean_e_d (e <= 2^-12.5), where n (1 - e) is as close; TIER 2:
 // beyond lean_soft_d (e <= 2^-25), where n alone is.
+// LJ of one very far pair into s.v (TIER 2: its attractive part into v2);
+// returns 1/d for the electrostatics
+template <int TIER>
+__device__ __forceinline__ float vfar_lj(const float4 &P4, const Slot &me, const PairParams &q, const DevScalars &sc,
+                                         Sums &s, float &v2)
+{
+    const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = __builtin_amdgcn_rsqf(d2);
+    const float r2 = rinv * rinv;
+    const float r3 = r2 * rinv;
+    const float r6 = r3 * r3;
+    if (TIER == 0) {
+        const float X = q.p1a * (r6 * r3);
+        const float n = fmaf(-q.p2a, r6, X);
+        const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
+        s.v += fmaf(-n, fmaf(-e, e, e), n); // n - n (e - e^2)
+    } else if (TIER == 1) {
+        const float X = q.p1a * (r6 * r3);
+        const float n = fmaf(-q.p2a, r6, X);
+        s.v = fmaf(n, fmaf(-X, fmaf(sc.lj_b, d2, sc.lj_a), 1.0f), s.v); // n (1 - e)
+    } else {
+        // n = X - c p2a r^6 as two sums: at this range the repulsive part is
+        // a small fraction of the attractive one (below 1e-4 for the shipped
+        // parameters), so they do not cancel
+        s.v = fmaf(q.p1a, r6 * r3, s.v);
+        v2 = fmaf(q.p2a, r6, v2);
+    }
+    return rinv;
+}
+
 template <int TIER>
 __device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, int pb, int pe, const Slot &me,
                                                 const PairParams &q, const DevScalars &sc, Sums &s)
 {
     const cfloat *ptc = (const cfloat *)ptc_;
     int p = pb;
-    float v2 = 0.0f; // TIER 2: the attractive part, summed apart
+    float v2 = 0.0f;
     for (; p + 4 <= pe; p += 4) {
         float ge = 0.0f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float4 P4 = ld_point(ptc, p + j);
-            const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
-            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            const float rinv = __builtin_amdgcn_rsqf(d2);
-            const float r2 = rinv * rinv;
-            const float r3 = r2 * rinv;
-            const float r6 = r3 * r3;
-            if (TIER == 0) {
-                const float X = q.p1a * (r6 * r3);
-                const float n = fmaf(-q.p2a, r6, X);
-                const float e = X * fmaf(sc.lj_b, d2, sc.lj_a);
-                s.v += fmaf(-n, fmaf(-e, e, e), n); // n - n (e - e^2)
-            } else if (TIER == 1) {
-                const float X = q.p1a * (r6 * r3);
-                const float n = fmaf(-q.p2a, r6, X);
-                s.v = fmaf(n, fmaf(-X, fmaf(sc.lj_b, d2, sc.lj_a), 1.0f), s.v); // n (1 - e)
-            } else {
-                // n = X - c p2a r^6 as two sums: at this range the repulsive
-                // part is a small fraction of the attractive one (below 1e-4
-                // for the shipped parameters), so they do not cancel
-                s.v = fmaf(q.p1a, r6 * r3, s.v);
-                v2 = fmaf(q.p2a, r6, v2);
-            }
-            ge = fmaf(P4.w, rinv, ge);
+            ge = fmaf(P4.w, vfar_lj<TIER>(P4, me, q, sc, s, v2), ge);
         }
         kahan_add(s.e, s.ec, ge);
     }
-    if (TIER == 2) s.v -= v2;
+    // the last points of the block, with the same arithmetic
     for (; p < pe; ++p) {
         const float4 P4 = ld_point(ptc, p);
-        const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float rinv = __builtin_amdgcn_rsqf(d2);
-        lj_term(d2, rinv, q, sc, s);
-        kahan_add(s.e, s.ec, P4.w * rinv);
+        kahan_add(s.e, s.ec, P4.w * vfar_lj<TIER>(P4, me, q, sc, s, v2));
     }
+    if (TIER == 2) s.v -= v2;
     s.p = fmaf(q.pmf1, (float)(pe - pb), s.p);
 }
 
