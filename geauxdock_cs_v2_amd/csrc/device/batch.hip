@@ -368,6 +368,8 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
     std::vector<Shape> shapes(n_jobs);
     if (opts->slots_per_wave > 0 || opts->point_slices > 0) {
         if (W <= 0) return fail(GD_ERR_ARG, "explicit shapes need waves_per_group");
+        if (W > 16 || (opts->point_slices > 0 && W % opts->point_slices))
+            return fail(GD_ERR_ARG, "waves_per_group must be at most 16 and a multiple of point_slices");
         for (int i = 0; i < n_jobs; ++i) {
             const int SW = std::max(1, W / std::max(1, opts->point_slices));
             if (W % SW || 64 * SW < js[i]->natom) return fail(GD_ERR_ARG, "shape does not fit the ligand");
@@ -380,6 +382,7 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
         W = shapes[0].waves;
     } else {
         if (W <= 0) W = 12;
+        if (W > 16) return fail(GD_ERR_ARG, "waves_per_group must be at most 16");
         // each job's share of the workgroups resident at once (two per CU)
         const int slots = std::max(1, 2 * std::max(ctx->cu_count, 1) / n_jobs);
         for (int i = 0; i < n_jobs; ++i) {
