@@ -114,6 +114,7 @@ struct Request {
     size_t footprint = 0;
     double t_submit = 0, t_admit = -1, t_done = -1;
     std::vector<gd_ReplicaMC> result;
+    std::string error; // why the request failed or was cancelled
 };
 
 } // namespace
@@ -143,7 +144,7 @@ struct gd_server {
     }
     void loop();
     void admit_locked();
-    void finish_locked(Request &q, int status);
+    void finish_locked(Request &q, int status, const std::string &why = std::string());
     double lane_load_locked(int k) const;
     void rebalance_locked();
     void prepare_locked(int k);
@@ -171,7 +172,7 @@ double gd_server::lane_load_locked(int k) const
     return w;
 }
 
-void gd_server::finish_locked(Request &q, int status)
+void gd_server::finish_locked(Request &q, int status, const std::string &why)
 {
     if (q.job_id >= 0) {
         gd_free_job(ctx, q.job_id);
@@ -179,6 +180,7 @@ void gd_server::finish_locked(Request &q, int status)
         used -= q.footprint;
     }
     q.status = status;
+    if (status != GD_OK) q.error = why;
     q.t_done = now();
     q.job = JobCopy(); // release host tables
 }
@@ -189,7 +191,7 @@ void gd_server::admit_locked()
         Request &q = reqs.at(pending.front());
         if (q.cancel) {
             pending.pop_front();
-            finish_locked(q, GD_ERR_STATE);
+            finish_locked(q, GD_ERR_STATE, "cancelled before admission");
             continue;
         }
         if (used + q.footprint > opts.hbm_budget) break; // wait for room, keep arrival order
@@ -197,7 +199,7 @@ void gd_server::admit_locked()
         int jid = -1;
         const int rc = gd_upload_job(ctx, target, &q.job.s.view, q.seed, &jid);
         if (rc != GD_OK) {
-            finish_locked(q, rc);
+            finish_locked(q, rc, std::string("admission: ") + gd_last_error());
             continue;
         }
         q.job_id = jid;
@@ -306,18 +308,17 @@ void gd_server::complete_locked(int k)
     for (size_t i = 0; i < L.ids.size(); ++i) {
         Request &q = reqs.at(L.ids[i]);
         if (e != hipSuccess) {
-            fail(GD_ERR_DEVICE, std::string("serving launch failed: ") + hipGetErrorString(e));
-            finish_locked(q, GD_ERR_DEVICE);
+            finish_locked(q, GD_ERR_DEVICE, std::string("serving launch failed: ") + hipGetErrorString(e));
             continue;
         }
         q.done += L.s2[i];
         q.rounds += L.exch[i];
         if (q.cancel) {
-            finish_locked(q, GD_ERR_STATE);
+            finish_locked(q, GD_ERR_STATE, "cancelled");
         } else if (q.done >= q.steps) {
             q.result.resize(ctx->jobs.at(q.job_id).n_rep);
             const int rc = gd_read_replicas(ctx, q.job_id, q.result.data());
-            finish_locked(q, rc);
+            finish_locked(q, rc, rc == GD_OK ? std::string() : std::string("read-back: ") + gd_last_error());
         }
     }
     std::vector<int> still;
@@ -339,10 +340,10 @@ void gd_server::loop()
         {
             std::unique_lock<std::mutex> lk(mu);
             if (!ok) { // no streams: every request fails
-                for (int id : active) finish_locked(reqs.at(id), GD_ERR_DEVICE);
+                for (int id : active) finish_locked(reqs.at(id), GD_ERR_DEVICE, "the serving streams could not be created");
                 active.clear();
                 while (!pending.empty()) {
-                    finish_locked(reqs.at(pending.front()), GD_ERR_DEVICE);
+                    finish_locked(reqs.at(pending.front()), GD_ERR_DEVICE, "the serving streams could not be created");
                     pending.pop_front();
                 }
                 cv.notify_all();
@@ -469,6 +470,8 @@ int gd_server_poll(gd_server *srv, int req_id, gd_request_result *res, gd_Replic
     }
     if (q.status == GD_OK && final_replicas && !q.result.empty())
         std::memcpy(final_replicas, q.result.data(), sizeof(gd_ReplicaMC) * q.result.size());
+    // a failed or cancelled request's reason becomes this thread's last error
+    if (q.status != GD_OK && q.status != GD_ERR_BUSY) (void)fail(q.status, q.error);
     return GD_OK;
 }
 

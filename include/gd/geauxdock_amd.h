@@ -213,6 +213,9 @@ int gd_server_create(gd_ctx *ctx, int target_id, const gd_server_opts *opts, gd_
 void gd_server_destroy(gd_server *srv);
 /* queue a request: the complex's job tables, steps, seed; returns an id */
 int gd_server_submit(gd_server *srv, const gd_Complex *cx, int steps, uint64_t seed, int *req_id);
+/* a request's status and times, and its final replicas once done; for a
+ * failed or cancelled request gd_last_error() on the calling thread then
+ * holds the reason */
 int gd_server_poll(gd_server *srv, int req_id, gd_request_result *res, gd_ReplicaMC *final_replicas);
 /* cancel a queued or running request; it finishes with GD_ERR_STATE */
 int gd_server_cancel(gd_server *srv, int req_id);

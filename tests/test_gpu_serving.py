@@ -114,6 +114,15 @@ def test_cancel_and_bad_requests_fail_alone(setup):
     L.check(lib.gd_server_submit(srv, C.byref(cx["A"].view), 200, 1, C.byref(a)))
     L.check(lib.gd_server_submit(srv, C.byref(cx["C"].view), 200, 2, C.byref(b)))
     L.check(lib.gd_server_cancel(srv, b.value))
+    # a request whose replica table breaks the ladder layout passes the size
+    # checks at submission and fails alone at admission, with its reason
+    reps = cx["B"].replicas()
+    n_lig_b = cx["B"].size["n_lig"]
+    reps[n_lig_b].idx_tmp = reps[0].idx_tmp  # ladder (0, 0) holds temperature 0 twice
+    bad_view = D.Complex.from_buffer_copy(cx["B"].view)
+    bad_view.replica = C.cast(reps, C.POINTER(D.ReplicaMC))
+    m = C.c_int()
+    L.check(lib.gd_server_submit(srv, C.byref(bad_view), 100, 3, C.byref(m)))
     # a request larger than the whole budget is refused at submission
     small = D.ServerOpts(1024, 1, 50, 0)
     srv2 = C.c_void_p()
@@ -122,9 +131,13 @@ def test_cancel_and_bad_requests_fail_alone(setup):
     assert lib.gd_server_submit(srv2, C.byref(cx["A"].view), 10, 1, C.byref(c)) == L.GD_ERR_NOMEM
     lib.gd_server_destroy(srv2)
     L.check(lib.gd_server_drain(srv))
-    ra, rb = D.RequestResult(), D.RequestResult()
+    ra, rb, rm = D.RequestResult(), D.RequestResult(), D.RequestResult()
     L.check(lib.gd_server_poll(srv, a.value, C.byref(ra), None))
     L.check(lib.gd_server_poll(srv, b.value, C.byref(rb), None))
+    why_b = lib.gd_last_error().decode()
+    L.check(lib.gd_server_poll(srv, m.value, C.byref(rm), None))
+    why_m = lib.gd_last_error().decode()
     lib.gd_server_destroy(srv)
     assert ra.status == 0
-    assert rb.status == L.GD_ERR_STATE
+    assert rb.status == L.GD_ERR_STATE and "cancelled" in why_b, why_b
+    assert rm.status == L.GD_ERR_ARG and "permutation" in why_m, why_m
