@@ -142,6 +142,8 @@ def run_c2(args, dev, perf, synth):
     dev.free_target(t)
 
     traffic = traffic_from_summary()
+    # the path is pairwise vector work with no matrix-core form: it is bound by
+    # the FP32 vector ALUs, whose peak equals MI355X's published FP32 figure
     roof = {"bound": "valu", "achieved": achieved, "peak": perf.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / perf.FP32_PEAK_TFLOPS, "traffic": traffic,
             "kernel": "mc_kernel", "chunk_ms": chunk_ms, "steps_per_chunk": steps_per_chunk,
