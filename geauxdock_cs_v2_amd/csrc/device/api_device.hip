@@ -58,6 +58,23 @@ Shape plain_shape(int natom)
     return s;
 }
 
+// The arrays of a complex that the uploads (target: the protein tables; job:
+// ligand, KDE, MCS, temperatures, replicas, PSP and the point topology that
+// is checked against the target) read; a null one is refused, not followed.
+int check_complex_arrays(const gd_Complex *cx, bool target, bool job)
+{
+    if (target && (!cx->prt_x || !cx->prt_y || !cx->prt_z || !cx->prt_hpp || !cx->prt_ele || !cx->prt_t || !cx->prt_c ||
+                   !cx->prt_cdc))
+        return fail(GD_ERR_ARG, "a protein table of the complex is null");
+    if (job && (!cx->lig_x2 || !cx->lig_y2 || !cx->lig_z2 || !cx->lig_c2 || !cx->lig_t2 || !cx->lig_x3 || !cx->lig_y3 ||
+                !cx->lig_z3 || !cx->lig_c3 || !cx->lig_t3 || !cx->kde_t || !cx->kde_x || !cx->kde_y || !cx->kde_z ||
+                !cx->mcs_ncol || !cx->mcs_tcc || !cx->mcs_rowptr || !cx->mcs_i || !cx->mcs_x || !cx->mcs_y ||
+                !cx->mcs_z || !cx->temp || !cx->replica || !cx->prt_t || !cx->prt_c || !cx->prt_seq3r ||
+                (cx->psp_width > 0 && !cx->psp)))
+        return fail(GD_ERR_ARG, "a job table of the complex is null");
+    return GD_OK;
+}
+
 int check_stream_range(const JobRes &j, long s_end)
 {
     if (j.kstream_steps > 0 && s_end - 1 > j.kstream_steps)
@@ -207,6 +224,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!cx || !target_id) return fail(GD_ERR_ARG, "null argument");
+    if (int rc = check_complex_arrays(cx, true, false)) return rc;
     const int np = cx->size.prt_npoint, nconf = cx->size.n_prt;
     if (np <= 0 || nconf <= 0) return fail(GD_ERR_ARG, "empty protein");
     const gd_EnePara &ep = cx->enepara;
@@ -517,6 +535,7 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!cx || !job_id) return fail(GD_ERR_ARG, "null argument");
+    if (int rc = check_complex_arrays(cx, false, true)) return rc;
     auto tit = ctx->targets.find(target_id);
     if (tit == ctx->targets.end()) return fail(GD_ERR_STATE, "unknown target");
     const gd_ComplexSize &z = cx->size;
@@ -527,7 +546,6 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
     if (z.prt_npoint != tit->second.npoint) return fail(GD_ERR_ARG, "job and target point counts differ");
     if (z.n_rep != z.n_lig * z.n_prt * z.n_tmp) return fail(GD_ERR_ARG, "n_rep != n_lig * n_prt * n_tmp");
     const int na = z.lig_natom;
-    if (!cx->replica) return fail(GD_ERR_ARG, "null replica table");
     if (int rc = check_replica_layout(cx->replica, z.n_lig, z.n_prt, z.n_tmp)) return rc;
 
     std::vector<float4> lig3((size_t)z.n_lig * na), lig2((size_t)z.n_lig * na);

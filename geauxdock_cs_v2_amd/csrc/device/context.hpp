@@ -104,6 +104,7 @@ int ensure_records(gd_ctx *ctx, JobRes &j, int cap);
 // fails unless the job's explicit stream words (if any) cover every step
 // before s_end (steps from 1 on draw words; step 0 draws none)
 int check_stream_range(const JobRes &j, long s_end);
+int check_complex_arrays(const gd_Complex *cx, bool target, bool job);
 int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
               int waves, int max_rep, bool compensated, bool records);
 // creates the context's group streams on first use

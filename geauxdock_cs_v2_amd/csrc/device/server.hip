@@ -433,6 +433,7 @@ void gd_server_destroy(gd_server *srv)
 int gd_server_submit(gd_server *srv, const gd_Complex *cx, int steps, uint64_t seed, int *req_id)
 {
     if (!srv || !cx || !req_id || steps <= 0) return fail(GD_ERR_ARG, "bad submit arguments");
+    if (int rc = check_complex_arrays(cx, false, true)) return rc;
     const gd_ComplexSize &z = cx->size;
     if (z.lig_natom <= 0 || z.lig_natom > 64 * kServerWaves || z.n_rep <= 0 || z.mcs_nrow <= 0 || z.kde_npoint <= 0)
         return fail(GD_ERR_ARG, "bad request sizes");

@@ -459,6 +459,32 @@ def test_bad_replica_layouts_are_refused(lib, ctx, kat_complex11):
     L.check(lib.gd_free_target(ctx, t))
 
 
+def test_null_tables_are_refused(lib, ctx, kat_complex11):
+    """A complex with a null table is refused with GD_ERR_ARG by the target
+    and job uploads and by serving submission, instead of being read."""
+    cx = kat_complex11[1]
+    t = C.c_int()
+    say("case: null protein, KDE and replica tables refused")
+    view = D.Complex.from_buffer_copy(cx.view)
+    view.prt_cdc = None
+    assert lib.gd_upload_target(ctx, C.byref(view), C.byref(t)) == L.GD_ERR_ARG
+    L.check(lib.gd_upload_target(ctx, C.byref(cx.view), C.byref(t)))
+    j = C.c_int()
+    for field in ("kde_x", "replica", "mcs_rowptr"):
+        view = D.Complex.from_buffer_copy(cx.view)
+        setattr(view, field, None)
+        assert lib.gd_upload_job(ctx, t, C.byref(view), 1, C.byref(j)) == L.GD_ERR_ARG, field
+        assert "null" in lib.gd_last_error().decode()
+    opts = D.ServerOpts(1 << 34, 4, 50, 0)
+    srv = C.c_void_p()
+    L.check(lib.gd_server_create(ctx, t, C.byref(opts), C.byref(srv)))
+    view = D.Complex.from_buffer_copy(cx.view)
+    view.temp = None
+    assert lib.gd_server_submit(srv, C.byref(view), 10, 1, C.byref(j)) == L.GD_ERR_ARG
+    lib.gd_server_destroy(srv)
+    L.check(lib.gd_free_target(ctx, t))
+
+
 def test_mclog_after_autotuned_batch(lib, ctx):
     """The autotuner's trial launches leave the job as they found it: after an
     autotuned batched chunk the McLog accept counters equal the chunk's
