@@ -82,6 +82,52 @@ void derive_enepara(const gd_EnePara0 &e0, gd_EnePara &e)
 
 } // namespace
 
+// A loader output built by a caller rather than by Loader::load: every array
+// the staging reads is present and every index it follows stays inside its
+// table (the loader itself only writes such values).
+static bool check_raw(const gd_Raw &raw, std::string &err)
+{
+    if (!raw.lig_x || !raw.lig_y || !raw.lig_z || !raw.lig_t || !raw.lig_c || !raw.prt_x || !raw.prt_y ||
+        !raw.prt_z || !raw.prt_t || !raw.prt_c || !raw.prt_d || !raw.prt_r || !raw.prt_seq3 || !raw.kde_t ||
+        !raw.kde_x || !raw.kde_y || !raw.kde_z || !raw.mcs_tcc || !raw.mcs_rowptr || !raw.mcs_i || !raw.mcs_x ||
+        !raw.mcs_y || !raw.mcs_z || (raw.psp_n > 0 && (!raw.psp_res || !raw.psp_lig || !raw.psp_val))) {
+        err = "a table of the loader output is null";
+        return false;
+    }
+    for (int j = 0; j < raw.lig_natom; ++j)
+        if (raw.lig_t[j] < 0 || raw.lig_t[j] >= GD_MAXTP2) {
+            err = "ligand atom type out of range";
+            return false;
+        }
+    for (int j = 0; j < raw.prt_npoint; ++j) {
+        const int t = raw.prt_t[j], d = raw.prt_d[j], r = raw.prt_r[j];
+        if (t < 0 || t >= GD_MAXTP1 - 1 || d < 0 || d >= GD_MAXTP4 || r < 0 || r >= raw.prt_nres) {
+            err = "protein point type, residue code or residue index out of range";
+            return false;
+        }
+    }
+    for (int k = 0; k < raw.psp_n; ++k)
+        if (raw.psp_res[k] < 0 || raw.psp_lig[k] < 0 || raw.psp_lig[k] >= GD_MAXTP2) {
+            err = "PSP entry out of range";
+            return false;
+        }
+    if (raw.mcs_rowptr[0] != 0) {
+        err = "MCS row pointers must start at 0";
+        return false;
+    }
+    for (int m = 0; m < raw.mcs_nrow; ++m)
+        if (raw.mcs_rowptr[m + 1] <= raw.mcs_rowptr[m]) {
+            err = "MCS rows must be non-empty and in order";
+            return false;
+        }
+    for (int k = 0; k < raw.mcs_rowptr[raw.mcs_nrow]; ++k)
+        if (raw.mcs_i[k] < 0 || raw.mcs_i[k] >= raw.lig_natom) {
+            err = "MCS atom index out of range";
+            return false;
+        }
+    return true;
+}
+
 bool stage(const gd_Raw &raw, StagedJob &out, std::string &err)
 {
     out = StagedJob();
@@ -93,6 +139,7 @@ bool stage(const gd_Raw &raw, StagedJob &out, std::string &err)
         err = "empty input: every size must be positive";
         return false;
     }
+    if (!check_raw(raw, err)) return false;
 
     // ---- protein ----
     std::vector<int> cdc(npt), seq3r(npt);

@@ -63,3 +63,47 @@ def test_docker_header_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{os.path.join(ROOT, 'include')}", str(src)],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0, r.stdout
+
+
+def test_staging_refuses_malformed_loader_output(lib, kat_complex):
+    """gd_stage checks a caller-built loader output before following it: null
+    tables and indices outside their tables are refused with GD_ERR_ARG (the
+    library's own loader never writes them).  CPU only."""
+    import numpy as np
+    raw = kat_complex[0]
+    assert raw.stage() is not None  # the loader's own output stages
+    h = C.c_void_p()
+
+    def refused(edit):
+        v = D.Raw.from_buffer_copy(raw.view)
+        keep = edit(v)
+        rc = lib.gd_stage(C.byref(v), C.byref(h))
+        del keep
+        return rc == L.GD_ERR_ARG, lib.gd_last_error().decode()
+
+    def null_lig_t(v):
+        v.lig_t = None
+
+    def bad_residue_index(v):
+        r = np.ctypeslib.as_array(v.prt_r, shape=(v.prt_npoint,)).copy()
+        r[0] = v.prt_nres
+        v.prt_r = r.ctypes.data_as(C.POINTER(C.c_int))
+        return r
+
+    def bad_point_type(v):
+        t = np.ctypeslib.as_array(v.prt_t, shape=(v.prt_npoint,)).copy()
+        t[3] = D.MAXTP1
+        v.prt_t = t.ctypes.data_as(C.POINTER(C.c_int))
+        return t
+
+    def bad_mcs_atom(v):
+        n = np.ctypeslib.as_array(v.mcs_rowptr, shape=(v.mcs_nrow + 1,))[-1]
+        i = np.ctypeslib.as_array(v.mcs_i, shape=(n,)).copy()
+        i[-1] = v.lig_natom
+        v.mcs_i = i.ctypes.data_as(C.POINTER(C.c_int))
+        return i
+
+    for edit, words in [(null_lig_t, "null"), (bad_residue_index, "residue index"),
+                        (bad_point_type, "point type"), (bad_mcs_atom, "MCS atom index")]:
+        ok, why = refused(edit)
+        assert ok and words in why, (edit.__name__, why)
