@@ -352,6 +352,8 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
     for (int i = 0; i < n_jobs; ++i) {
         auto it = ctx->jobs.find(job_ids[i]);
         if (it == ctx->jobs.end()) return fail(GD_ERR_STATE, "unknown job");
+        if (std::find(js.begin(), js.end(), &it->second) != js.end())
+            return fail(GD_ERR_ARG, "a job is listed twice"); // its replicas would be advanced twice at once
         if (target >= 0 && it->second.target != target) return fail(GD_ERR_ARG, "batched jobs must share a target");
         target = it->second.target;
         js.push_back(&it->second);

@@ -288,6 +288,10 @@ def test_two_ligands_one_resident_target(lib, ctx, tmp_path):
         L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), 100 + i, C.byref(j)))
         jobs.append(j.value)
     say("case: two ligands, one target, batched 40 steps")
+    # a job listed twice would have its replicas advanced twice at once
+    o = D.BatchOpts(10, 0, 0, 0, 8, 0, 1, 0)
+    dup = (C.c_int * 3)(jobs[0], jobs[1], jobs[0])
+    assert lib.gd_batch_run(ctx, dup, 3, C.byref(o), None) == L.GD_ERR_ARG
     _batch(lib, ctx, jobs, 40, waves=8)
     for cx, j in zip(cxs, jobs):
         n = cx.size["n_rep"]
