@@ -169,18 +169,22 @@ class Docker:
         cx = self.complex
         n = cx.size["n_rep"]
         mc = cx.view.mcpara
+        if mc.steps_per_dump <= 0 or mc.steps_total < 0:
+            raise ValueError("Docker.run: bad step counts")
         t = self.dev.upload_target(cx)
-        j = self.dev.upload_job(t, cx, self.seed)
         try:
-            rnd = 0
-            for s1 in range(0, mc.steps_total, mc.steps_per_dump):
-                s2 = min(mc.steps_per_dump, mc.steps_total - s1)
-                self.dev.montecarlo(j, s1, s2)
-                self.record, self.next_entry = self.dev.records(j, n, max(s2, 100))
-                if self.exchange and cx.size["n_tmp"] > 1:
-                    self.dev.exchange(j, rnd)
-                    rnd += 1
-            self.replica = self.dev.replicas(j, n)
+            j = self.dev.upload_job(t, cx, self.seed)
+            try:
+                rnd = 0
+                for s1 in range(0, mc.steps_total, mc.steps_per_dump):
+                    s2 = min(mc.steps_per_dump, mc.steps_total - s1)
+                    self.dev.montecarlo(j, s1, s2)
+                    self.record, self.next_entry = self.dev.records(j, n, max(s2, 100))
+                    if self.exchange and cx.size["n_tmp"] > 1:
+                        self.dev.exchange(j, rnd)
+                        rnd += 1
+                self.replica = self.dev.replicas(j, n)
+            finally:
+                self.dev.free_job(j)
         finally:
-            self.dev.free_job(j)
             self.dev.free_target(t)

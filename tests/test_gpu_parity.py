@@ -511,6 +511,24 @@ def test_mclog_after_autotuned_batch(lib, ctx):
     assert np.array_equal(mc, ne - 1)
 
 
+def test_python_docker_known_answer(lib, kat_complex):
+    """The Python Docker (the reference's Docker{complex, record, run()}) on
+    the 1a07C job: one 100-step chunk whose first record entry of replica 0 is
+    the known-answer step-0 energies, and the final replica states read back."""
+    from geauxdock_cs_v2_amd.device import Device, Docker
+    say("case: Python Docker on 1a07C")
+    with Device(0) as dev:
+        d = Docker(dev, seed=7)
+        d.complex = kat_complex[1]
+        d.run()
+        n = kat_complex[1].size["n_rep"]
+        first = d.record[0]
+        assert first.step == 0
+        assert np.all(np.abs(np.array(first.energy[:10]) - KAT_1A07C1) <= 5e-5), list(first.energy)
+        assert len(d.replica) == n and all(1 <= d.next_entry[r] <= 100 for r in range(n))
+        assert all(d.replica[r].step == d.record[r * 100 + d.next_entry[r] - 1].step for r in range(n))
+
+
 def test_docking_driver_known_answer(lib, tmp_path):
     """The native driver (the reference's 3_geauxdock over gd::Docker) on a CSV
     job list holding the 1a07C job: its first PrintCsv line is the
