@@ -1,72 +1,91 @@
 """Benchmark: replica-MC pose evaluations/s on one MI355X.
 
-Headline workload (`value`): the full replica ensemble -- a 48-atom ligand in
-20 conformations against a 4,000-point protein in 11 conformations, KDE
-1,000, MCS 40 x 24, 16 temperatures (0.3-2.0): 3,520 replicas, exchange
-between neighbouring temperatures every 100 steps.  Synthetic inputs are
-written in the SDF/PDB/.ff formats from one seed and read through the loader.
-One "step" is one MC step of every replica (3,520 pose evaluations).  The
-timed region runs the batched form (replicas packed per workgroup, autotuned
-shape, hipGraph-replayed 100-step chunks + exchange), inputs resident in HBM.
+Headline workload (`value`, `config`, `roofline`, `cpu_baseline`): the
+largest single-GPU configuration, the stress shape -- a 150-atom ligand in
+64 conformations against a 20,000-point protein (35 A sphere) in 32
+conformations, KDE 5,000, MCS 100 rows, 32 temperatures: 65,536 replicas,
+exchange between neighbouring temperatures every 50 steps.  Synthetic inputs
+are written in the SDF/PDB/.ff formats from one seed and read through the
+loader.  One "step" is one MC step of every replica (65,536 pose
+evaluations).  The timed region runs the batched form (persistent fused
+kernel over a cost-ordered work queue, autotuned workgroup shape,
+hipGraph-replayed chunks + exchange) with inputs resident in HBM; `--steps`
+steps after `--warmup` untimed steps (the autotuner's trial launches run in
+the warm-up).
 
-Beside it, each in its own section of the JSON line:
-  plain     the plain form (one workgroup per replica, one launch per
-            100-step chunk, record read-back per chunk) over the same steps;
-  roofline  the Monte Carlo kernel's algorithmic FLOP rate: FLOP per pose x
-            poses per 100-step chunk / the average chunk time of the timed
-            region (hipEvents around the chunk sequence; a chunk is the
-            kernel's dispatches for the two workgroup groups, which run side
-            by side on two streams); traffic per chunk from the committed
-            rocprofv3 counter summary;
+Beside it, each in its own section of the JSON line, each with its own
+fraction of the FP32 vector peak:
+  plain     the headline workload in the plain form (one workgroup per
+            replica, one launch per chunk, record read-back) over the same
+            steps;
+  c2        the full replica ensemble (48 atoms x 20 conformations against
+            4,000 points x 11 conformations, 16 temperatures: 3,520
+            replicas), batched and plain over the same steps, with the same
+            exchange rounds in both forms;
   screen    batched screening: 1,024 ligands (lognormal 12-96 atoms, 8
             conformations, 8 temperatures; ~393k replicas) against one
             resident 5,000-point protein in 6 conformations, all jobs in one
-            chunk of fused launches, on a window of steps; versus plain launches
-            (one job after another, one workgroup per replica) on a sample of
-            the ligands;
+            persistent launch per chunk; versus plain launches (one job after
+            another) on a sample of the ligands;
   serve     the serving loop on an 8,000-point protein in 4 conformations:
             requests of 10-120 atoms, 4-32 conformations, 8 temperatures,
             500-5,000 steps arriving as a Poisson stream at 50, 200 and 800
-            per second; p50/p99 job latency and achieved pose evaluations/s
-            per rate, versus the same requests run one after another with
-            plain launches (their throughput, and the latencies the same
-            arrivals would see);
-  stress    the stress shape at full size (65,536 replicas, 150 atoms,
-            20,000-point protein in 32 conformations) on a 20-step window,
-            batched and plain;
-  cpu_baseline  the CPU oracle (OpenMP restatement of MonteCarlo_d) on a
-            bounded sample of the headline workload.
+            per second, admitted under a slot limit that binds; p50/p99 job
+            latency, the queueing share of latency (admission - submission)
+            and achieved pose evaluations/s per rate, versus the same
+            requests run one after another with plain launches;
+  cpu_baseline  the CPU oracle (OpenMP restatement of MonteCarlo_d,
+            OMP_PROC_BIND=true) on a bounded sample of the headline workload.
+
+The roofline's `achieved` is algorithmic FLOP per launch (FLOP per pose x
+replicas x steps per launch) over the average launch duration of the Monte
+Carlo kernel in the timed region, measured with hipEvents on the stream the
+kernel is launched on; `traffic` is HBM bytes per launch from the committed
+rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this same command.
 """
-import argparse
-import json
-import math
 import os
-import re
-import sys
-import time
+
+# OpenMP reads its binding when the runtime loads (the CPU oracle's); the
+# reference's run instructions bind threads (doc/instructions.txt:140-141)
+os.environ.setdefault("OMP_PROC_BIND", "true")
+
+import argparse  # noqa: E402
+import json  # noqa: E402
+import math  # noqa: E402
+import re  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "replica-MC pose evaluations/s on 1 MI355X vs plain; p50/p99 job latency"
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "mc_kernel_c2_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "mc_kernel_stress_traffic.md")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cx, target_seconds):
+def frac(tflops, perf):
+    return None if tflops is None else tflops / perf.FP32_PEAK_TFLOPS
+
+
+def cpu_baseline(cx, target_seconds, sample_reps):
     """The CPU oracle (the repository's C restatement of MonteCarlo_d, OpenMP
-    over replicas) on a bounded sample: every 10th replica for as many steps
-    as fit the time budget."""
+    over replicas) on a bounded sample of the workload: `sample_reps`
+    replicas spread evenly over the ensemble for as many steps as fit the
+    time budget, timed on the host's cores."""
     from geauxdock_cs_v2_amd import ctypes_defs as D
     from geauxdock_cs_v2_amd import perf
     from oracle import bind as O
+    # host facts before the OpenMP runtime binds this thread to one CPU
+    host = perf.host_info()
     threads = perf.host_threads()
     reps = cx.replicas()
     n = len(reps)
-    sub_idx = list(range(0, n, 10))
+    stride = max(1, n // sample_reps)
+    sub_idx = list(range(0, n, stride))[:sample_reps]
     sub = (D.ReplicaMC * len(sub_idx))(*[reps[i] for i in sub_idx])
     view = cx.view
     t0 = time.perf_counter()
@@ -77,91 +96,105 @@ def cpu_baseline(cx, target_seconds):
     O.montecarlo(view, sub, 1, steps, 99, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": len(sub_idx) * steps / dt, "unit": "pose evaluations/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sub_idx)} of {n} replicas (every 10th) x {steps} steps, {dt:.1f} s, "
-                      f"{threads} threads on {perf.cpu_model()}"}
+            "sample": f"{len(sub_idx)} of {n} replicas (every {stride}th) x {steps} steps, {dt:.1f} s, "
+                      f"{threads} threads (OMP_PROC_BIND={host['omp_proc_bind']}; affinity mask "
+                      f"{host['affinity_cpus']} CPUs, capped at {host['thread_cap']}, one GPU's host share) on "
+                      f"{host['sockets']} x {host['model']} ({host['cores_per_socket']} cores/socket, "
+                      f"{host['threads_per_core']} threads/core)",
+            "host": host}
 
 
-def traffic_from_summary():
-    """HBM bytes per 100-step chunk of the headline kernel, as the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary states it (None if absent)."""
+def traffic_from_summary(steps_per_launch):
+    """HBM bytes per launch of the headline kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary, with the steps per launch it was
+    measured at; scaled to `steps_per_launch` (None if absent)."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
-            m = re.search(r"^bytes_per_chunk:\s*([0-9.eE+]+)", f.read(), re.M)
-        return float(m.group(1)) if m else None
+            text = f.read()
     except OSError:
-        return None
+        return None, None
+    m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    k = re.search(r"^steps_per_launch:\s*([0-9]+)", text, re.M)
+    if not m or not k:
+        return None, None
+    b, s = float(m.group(1)), int(k.group(1))
+    return b * steps_per_launch / s, s
 
 
-def run_c2(args, dev, perf, synth):
-    cx, info = synth.load_config("c2")
-    cfg = info["cfg"]
-    n_rep = cx.size["n_rep"]
-    exch = cfg["exchange"]
-    chunk = exch if exch > 0 else 100
-    f_pose = perf.flops_per_pose(cx)
-    log(f"c2: {cx.size}, {f_pose} FLOP/pose")
-    shape = {}
-    if args.shape:
-        w, s = (int(x) for x in args.shape.split(","))
-        shape = {"waves": w, "point_slices": s}
-    t = dev.upload_target(cx)
-    j = dev.upload_job(t, cx, seed=2024)
-    # warm-up: the autotuner's trial launches (unless a shape is given) and
-    # one chunk of the real sequence
-    dev.batch_run([j], args.warmup, exchange=exch if args.warmup >= exch > 0 else 0, graph=True, **shape)
+def batched_window(dev, jobs, steps, warmup, exchange, shape=None):
+    """Warm-up (autotuner trials and `warmup` steps), then `steps` timed
+    steps; returns (wall s, device ms, launches, steps per launch)."""
+    shape = shape or {}
+    if warmup > 0:
+        dev.batch_run(jobs, warmup, exchange=exchange if warmup >= exchange > 0 else 0, graph=True, **shape)
     t0 = time.perf_counter()
-    dev_ms = dev.batch_run([j], args.steps, exchange=exch, graph=True, **shape)
+    dev_ms = dev.batch_run(jobs, steps, exchange=exchange, graph=True, **shape)
     wall = time.perf_counter() - t0
-    shape_used = dev.job_shape(j)
-    value = n_rep * args.steps / wall
-    chunks = math.ceil(args.steps / chunk)
-    # a chunk: the Monte Carlo kernel over every replica (one dispatch per
-    # workgroup group, side by side), then exchange and step advance (~10 us)
-    chunk_ms = dev_ms / chunks
-    steps_per_chunk = args.steps / chunks
-    achieved = f_pose * n_rep * steps_per_chunk / (chunk_ms * 1e-3) / 1e12
-    log(f"c2 batched: {args.steps} steps in {wall:.3f} s wall ({dev_ms:.1f} ms device, {chunks} chunks "
-        f"of {chunk_ms:.2f} ms), {value:.4g} poses/s, {achieved:.1f} TFLOP/s")
+    chunk = exchange if exchange > 0 else min(steps, 100)
+    launches = math.ceil(steps / chunk)
+    return wall, dev_ms, launches, steps / launches
+
+
+def plain_window(dev, t, cx, steps, exchange, seed):
+    """The plain form over `steps` steps: one launch per chunk (the exchange
+    interval, else 100 steps), records read back after each, an exchange
+    round after each full interval -- the rounds the batched form runs."""
+    n_rep = cx.size["n_rep"]
+    chunk = exchange if exchange > 0 else 100
+    jp = dev.upload_job(t, cx, seed=seed)
+    dev.montecarlo(jp, 0, 1)  # first-launch costs out of the window
+    rnd = 0
+    t0 = time.perf_counter()
+    s1 = 1
+    while s1 <= steps:
+        s2 = min(chunk - (s1 - 1) % chunk, steps + 1 - s1)
+        dev.montecarlo(jp, s1, s2)
+        dev.records(jp, n_rep, max(s2, 100))
+        s1 += s2
+        if exchange > 0 and (s1 - 1) % exchange == 0:
+            dev.exchange(jp, rnd)
+            rnd += 1
+    wall = time.perf_counter() - t0
+    dev.free_job(jp)
+    return n_rep * steps / wall
+
+
+def describe(cx, name, exch):
+    z = cx.size
+    return (f"{name}: ligand {z['lig_natom']} atoms x {z['n_lig']} conf, protein {z['prt_npoint']} points x "
+            f"{z['n_prt']} conf, {z['n_tmp']} temperatures, {z['n_rep']} replicas, exchange every {exch}")
+
+
+def run_ensemble(args, dev, perf, synth, name, steps, warmup, seed):
+    """One single-job configuration, batched then plain over the same steps."""
+    cx, info = synth.load_config(name)
+    n_rep = cx.size["n_rep"]
+    exch = info["cfg"]["exchange"]
+    f_pose = perf.flops_per_pose(cx)
+    log(f"{name}: {cx.size}, {f_pose} FLOP/pose")
+    t = dev.upload_target(cx)
+    j = dev.upload_job(t, cx, seed=seed)
+    wall, dev_ms, launches, spl = batched_window(dev, [j], steps, warmup, exch)
+    shape = dev.job_shape(j)
     dev.free_job(j)
-
-    plain_value = None
-    if args.plain_steps > 0:
-        jp = dev.upload_job(t, cx, seed=2024)
-        dev.montecarlo(jp, 0, 1)  # first-launch costs out of the window
-        t0 = time.perf_counter()
-        for k, s1 in enumerate(range(1, args.plain_steps + 1, 100)):
-            s2 = min(100, args.plain_steps + 1 - s1)
-            dev.montecarlo(jp, s1, s2)
-            dev.records(jp, n_rep, 100)
-            if exch:
-                dev.exchange(jp, k)
-        plain_wall = time.perf_counter() - t0
-        plain_value = n_rep * args.plain_steps / plain_wall
-        dev.free_job(jp)
-        log(f"c2 plain: {args.plain_steps} steps in {plain_wall:.3f} s, {plain_value:.4g} poses/s")
+    value = n_rep * steps / wall
+    launch_ms = dev_ms / launches
+    tflops = f_pose * n_rep * spl / (launch_ms * 1e-3) / 1e12
+    log(f"{name} batched: {steps} steps in {wall:.3f} s wall ({dev_ms:.1f} ms device, {launches} launches of "
+        f"{launch_ms:.2f} ms), {value:.4g} poses/s, {tflops:.1f} TFLOP/s")
+    plain = None
+    if args.plain:
+        plain = plain_window(dev, t, cx, steps, exch, seed)
+        log(f"{name} plain: {steps} steps, {plain:.4g} poses/s")
     dev.free_target(t)
-
-    traffic = traffic_from_summary()
-    # the path is pairwise vector work with no matrix-core form: it is bound by
-    # the FP32 vector ALUs, whose peak equals MI355X's published FP32 figure
-    roof = {"bound": "valu", "achieved": achieved, "peak": perf.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / perf.FP32_PEAK_TFLOPS, "traffic": traffic,
-            "kernel": "mc_kernel", "chunk_ms": chunk_ms, "steps_per_chunk": steps_per_chunk,
-            "unit_of_work": "one 100-step chunk of all replicas: the kernel's dispatches for the two "
-                            "workgroup groups, run side by side on two streams",
-            "flop_per_pose": f_pose,
-            "shape": {"waves": shape_used[0], "slot_waves": shape_used[1], "point_slices": shape_used[2],
-                      "replicas_per_group": shape_used[3], "chosen_by": "given" if args.shape else "autotuner"}}
-    if traffic:
-        roof["hbm_gbs"] = traffic / (chunk_ms * 1e-3) / 1e9
-        roof["hbm_frac"] = roof["hbm_gbs"] / perf.HBM_PEAK_GBS
-        roof["traffic_source"] = os.path.relpath(TRAFFIC_SUMMARY, ROOT)
-    workload = (f"c2: ligand {cx.size['lig_natom']} atoms x {cx.size['n_lig']} conf, protein "
-                f"{cx.size['prt_npoint']} points x {cx.size['n_prt']} conf, {cx.size['n_tmp']} temperatures, "
-                f"{n_rep} replicas, exchange every {exch}")
-    return cx, {"value": value, "wall": wall, "workload": workload, "n_rep": n_rep, "roofline": roof,
-                "plain": {"value": plain_value, "steps": args.plain_steps,
-                          "form": "one workgroup per replica, 100-step launches, record read-back per chunk"}}
+    return cx, {"workload": describe(cx, name, exch), "replicas": n_rep, "steps": steps, "value": value,
+                "wall_s": wall, "device_ms": dev_ms, "launches": launches, "steps_per_launch": spl,
+                "launch_ms": launch_ms, "flop_per_pose": f_pose, "tflops": tflops, "frac": frac(tflops, perf),
+                "shape": {"waves": shape[0], "slot_waves": shape[1], "point_slices": shape[2],
+                          "replicas_per_group": shape[3], "chosen_by": "autotuner"},
+                "plain": {"value": plain, "steps": steps,
+                          "form": "one workgroup per replica, one launch per chunk, record read-back per chunk"},
+                "speedup_vs_plain": value / plain if plain else None}
 
 
 def run_screen(args, dev, perf, synth):
@@ -172,14 +205,15 @@ def run_screen(args, dev, perf, synth):
     n_rep = sum(c.size["n_rep"] for c in cxs)
     t = dev.upload_target(cxs[0])
     jobs = [dev.upload_job(t, c, seed=100 + i) for i, c in enumerate(cxs)]
-    dev.batch_run(jobs, 10, graph=True)  # warm-up
+    dev.batch_run(jobs, 10, graph=True)  # warm-up (and the shape autotuner's trials)
     t0 = time.perf_counter()
     dev_ms = dev.batch_run(jobs, args.screen_steps, graph=True)
     wall = time.perf_counter() - t0
     value = n_rep * args.screen_steps / wall
     flop = sum(perf.flops_per_pose(c) * c.size["n_rep"] for c in cxs) * args.screen_steps
+    tflops = flop / (dev_ms * 1e-3) / 1e12
     log(f"screen batched: {args.screen_steps} steps x {n_rep} replicas in {wall:.3f} s "
-        f"({dev_ms:.1f} ms device), {value:.4g} poses/s, {flop / (dev_ms * 1e-3) / 1e12:.1f} TFLOP/s")
+        f"({dev_ms:.1f} ms device), {value:.4g} poses/s, {tflops:.1f} TFLOP/s")
     for j in jobs:
         dev.free_job(j)
     # plain launches, one ligand after another, on every 64th ligand
@@ -204,7 +238,7 @@ def run_screen(args, dev, perf, synth):
                         f"{sorted(atoms)[len(atoms) // 2]}) x 8 conf x 8 temperatures against a "
                         f"{cxs[0].size['prt_npoint']}-point protein x {cxs[0].size['n_prt']} conf, "
                         f"{n_rep} replicas",
-            "value": value, "steps": args.screen_steps, "tflops": flop / (dev_ms * 1e-3) / 1e12,
+            "value": value, "steps": args.screen_steps, "tflops": tflops, "frac": frac(tflops, perf),
             "plain": {"value": plain_value, "sample": f"ligands 0, 64, ... ({len(sample)}) x "
                                                       f"{args.screen_plain_steps} steps, one after another"},
             "speedup_vs_plain": value / plain_value}
@@ -216,6 +250,7 @@ def run_serve(args, dev, perf, synth, Server):
     cxs = synth.load_many(info, threads=perf.host_threads())
     steps = [l["steps"] for l in info["ligands"]]
     poses = [c.size["n_rep"] * s for c, s in zip(cxs, steps)]
+    flops = [perf.flops_per_pose(c) * p for c, p in zip(cxs, poses)]
     t = dev.upload_target(cxs[0])
     # serial plain launches of the same requests, every step of each: one
     # 100-step launch per chunk, record read-back, exchange every 100 steps
@@ -241,7 +276,8 @@ def run_serve(args, dev, perf, synth, Server):
     rng = np.random.default_rng(7)
     for lam in synth.SERVE["rates"]:
         arrivals = np.cumsum(rng.exponential(1.0 / lam, len(cxs)))
-        srv = Server(dev, t, hbm_budget=args.serve_hbm_gib << 30, max_active=64, chunk=100, exchange=100)
+        srv = Server(dev, t, hbm_budget=args.serve_hbm_gib << 30, max_active=args.serve_max_active, chunk=100,
+                     exchange=100)
         ids = []
         t0 = time.perf_counter()
         for i, (c, a) in enumerate(zip(cxs, arrivals)):
@@ -255,8 +291,10 @@ def run_serve(args, dev, perf, synth, Server):
         if any(r.status != 0 for r in res):
             raise RuntimeError(f"serving at {lam}/s: failed requests {[r.status for r in res]}")
         lat = np.array([r.t_done_ms - r.t_submit_ms for r in res])
+        queue = np.array([r.t_admit_ms - r.t_submit_ms for r in res])
         span = max(r.t_done_ms for r in res) - min(r.t_submit_ms for r in res)
         achieved = sum(poses) / (span * 1e-3)
+        tflops = sum(flops) / (span * 1e-3) / 1e12
         # the same arrivals served one after another by plain launches
         # (measured durations, first come first served)
         free_at, plain_lat = 0.0, []
@@ -264,76 +302,44 @@ def run_serve(args, dev, perf, synth, Server):
             start = max(a, free_at)
             free_at = start + dur
             plain_lat.append((free_at - a) * 1e3)
-        rates.append({"rate_per_s": lam, "requests": len(ids), "p50_ms": float(np.percentile(lat, 50)),
-                      "p99_ms": float(np.percentile(lat, 99)), "value": achieved,
+        rates.append({"rate_per_s": lam, "requests": len(ids), "max_active": args.serve_max_active,
+                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                      "queue_p50_ms": float(np.percentile(queue, 50)), "queue_p99_ms": float(np.percentile(queue, 99)),
+                      "queue_share": float(queue.sum() / lat.sum()),
+                      "value": achieved, "tflops": tflops, "frac": frac(tflops, perf),
                       "offered": lam * float(np.mean(poses)), "speedup_vs_plain": achieved / plain_value,
                       "plain_p50_ms": float(np.percentile(plain_lat, 50)),
                       "plain_p99_ms": float(np.percentile(plain_lat, 99))})
-        log(f"serve {lam:g}/s: p50 {rates[-1]['p50_ms']:.0f} ms, p99 {rates[-1]['p99_ms']:.0f} ms, "
-            f"{achieved:.4g} poses/s (offered {rates[-1]['offered']:.4g}); serial plain p50 "
-            f"{rates[-1]['plain_p50_ms']:.0f} ms, p99 {rates[-1]['plain_p99_ms']:.0f} ms")
+        log(f"serve {lam:g}/s: p50 {rates[-1]['p50_ms']:.0f} ms, p99 {rates[-1]['p99_ms']:.0f} ms, queueing "
+            f"{100 * rates[-1]['queue_share']:.0f}% of latency, {achieved:.4g} poses/s ({tflops:.1f} TFLOP/s, "
+            f"offered {rates[-1]['offered']:.4g}); serial plain p50 {rates[-1]['plain_p50_ms']:.0f} ms, "
+            f"p99 {rates[-1]['plain_p99_ms']:.0f} ms")
     dev.free_target(t)
     return {"workload": f"serve: {len(cxs)} requests per rate, {min(l['n_atoms'] for l in info['ligands'])}-"
                         f"{max(l['n_atoms'] for l in info['ligands'])} atoms, 4-32 conf, 8 temperatures, "
                         f"{min(steps)}-{max(steps)} steps, against an {cxs[0].size['prt_npoint']}-point protein "
-                        f"x {cxs[0].size['n_prt']} conf",
+                        f"x {cxs[0].size['n_prt']} conf, at most {args.serve_max_active} requests admitted at once",
             "rates": rates, "plain": {"value": plain_value, "sample": "every step of each request, one request "
                                                                        "after another, plain launches"}}
-
-
-def run_stress(args, dev, perf, synth):
-    """The stress shape at full size: 65,536 replicas of a 150-atom ligand
-    against a 20,000-point protein in 32 conformations, on a window of steps;
-    batched against plain launches over the same steps."""
-    cx, info = synth.load_config("c5")
-    n_rep = cx.size["n_rep"]
-    f_pose = perf.flops_per_pose(cx)
-    steps = args.stress_steps
-    t = dev.upload_target(cx)
-    j = dev.upload_job(t, cx, seed=5)
-    dev.batch_run([j], 1, graph=False)  # warm-up: the autotuner's trials (one step each at this size)
-    t0 = time.perf_counter()
-    dev_ms = dev.batch_run([j], steps, exchange=info["cfg"]["exchange"], graph=True)
-    wall = time.perf_counter() - t0
-    value = n_rep * steps / wall
-    shape = dev.job_shape(j)
-    dev.free_job(j)
-    jp = dev.upload_job(t, cx, seed=5)
-    dev.montecarlo(jp, 0, 1)
-    t0 = time.perf_counter()
-    dev.montecarlo(jp, 1, steps)
-    dev.records(jp, n_rep, max(steps, 100))
-    plain_wall = time.perf_counter() - t0
-    dev.free_job(jp)
-    dev.free_target(t)
-    plain_value = n_rep * steps / plain_wall
-    tflops = f_pose * n_rep * steps / (dev_ms * 1e-3) / 1e12
-    log(f"stress batched: {steps} steps x {n_rep} replicas in {wall:.3f} s, {value:.4g} poses/s, {tflops:.1f} TFLOP/s; "
-        f"plain {plain_value:.4g} poses/s")
-    return {"workload": f"stress: ligand {cx.size['lig_natom']} atoms x {cx.size['n_lig']} conf, protein "
-                        f"{cx.size['prt_npoint']} points x {cx.size['n_prt']} conf, {cx.size['n_tmp']} temperatures, "
-                        f"{n_rep} replicas", "steps": steps, "value": value, "tflops": tflops,
-            "shape": {"waves": shape[0], "slot_waves": shape[1], "point_slices": shape[2],
-                      "replicas_per_group": shape[3]},
-            "plain": {"value": plain_value, "steps": steps}, "speedup_vs_plain": value / plain_value}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--plain-steps", type=int, default=-1,
-                    help="steps of the plain-form comparison run (default: the same steps as the batched run)")
-    ap.add_argument("--sections", default="c2,screen,serve,stress",
-                    help="comma list of c2, screen, serve, stress (c2 gives value and roofline)")
-    ap.add_argument("--shape", default="", help="W,S: fixed workgroup shape for c2 (skips the autotuner)")
+    ap.add_argument("--steps", type=int, default=20, help="timed MC steps of the headline workload")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed MC steps before the timed ones")
+    ap.add_argument("--sections", default="stress,c2,screen,serve",
+                    help="comma list of stress (the headline), c2, screen, serve")
+    ap.add_argument("--c2-steps", type=int, default=-1, help="steps of the c2 section (default: --steps)")
+    ap.add_argument("--no-plain", dest="plain", action="store_false", help="skip the plain-form comparisons")
     ap.add_argument("--screen-steps", type=int, default=100)
     ap.add_argument("--screen-plain-steps", type=int, default=100)
-    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-max-active", type=int, default=4,
+                    help="requests admitted at once (binds: the rest queue until a slot frees)")
     ap.add_argument("--serve-hbm-gib", type=int, default=64)
-    ap.add_argument("--stress-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-replicas", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (scales rocprof byte counters)")
@@ -353,41 +359,58 @@ def main():
               "(replica ladders shard by job across GPUs with no collective in the MC loop)", file=sys.stderr)
         sys.exit(2)
     sections = set(s.strip() for s in args.sections.split(",") if s.strip())
-    # without c2 (profiling runs of one section) the line has no headline value
+    if args.c2_steps < 0:
+        args.c2_steps = args.steps
 
-    if args.plain_steps < 0:
-        args.plain_steps = args.steps
     from geauxdock_cs_v2_amd import perf, synth
     from geauxdock_cs_v2_amd.device import Device, Server
 
     dev = Device(0)
-    cx, c2 = run_c2(args, dev, perf, synth) if "c2" in sections else (None, None)
+    cx5, stress = (run_ensemble(args, dev, perf, synth, "c5", args.steps, args.warmup, 5)
+                   if "stress" in sections else (None, None))
+    c2 = run_ensemble(args, dev, perf, synth, "c2", args.c2_steps, args.warmup, 2024)[1] if "c2" in sections else None
     screen = run_screen(args, dev, perf, synth) if "screen" in sections else None
     serve = run_serve(args, dev, perf, synth, Server) if "serve" in sections else None
-    stress = run_stress(args, dev, perf, synth) if "stress" in sections else None
     dev.close()
 
-    cpu = None if (args.no_cpu or cx is None) else cpu_baseline(cx, args.cpu_seconds)
-    if cpu:
+    cpu = None
+    if not args.no_cpu and cx5 is not None:
+        cpu = cpu_baseline(cx5, args.cpu_seconds, args.cpu_replicas)
         log(f"cpu oracle: {cpu['value']:.4g} poses/s ({cpu['sample']})")
 
-    c2 = c2 or {"value": None, "wall": float("nan"), "workload": None, "n_rep": None, "roofline": None,
-                "plain": {"value": None}}
-    value = c2["value"]
-    plain = c2["plain"]
+    roof = None
+    if stress:
+        traffic, traffic_steps = traffic_from_summary(stress["steps_per_launch"])
+        # pairwise vector work with no matrix-core form: bound by the FP32
+        # vector ALUs, whose peak is MI355X's published FP32 figure
+        roof = {"bound": "valu", "achieved": stress["tflops"], "peak": perf.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": stress["frac"], "traffic": traffic, "kernel": "mc_kernel",
+                "launch_ms": stress["launch_ms"], "launches": stress["launches"],
+                "steps_per_launch": stress["steps_per_launch"],
+                "unit_of_work": f"one launch of the Monte Carlo kernel: {stress['steps_per_launch']:g} steps of all "
+                                f"{stress['replicas']} replicas",
+                "flop_per_pose": stress["flop_per_pose"], "shape": stress["shape"]}
+        if traffic:
+            roof["hbm_gbs"] = traffic / (stress["launch_ms"] * 1e-3) / 1e9
+            roof["hbm_frac"] = roof["hbm_gbs"] / perf.HBM_PEAK_GBS
+            roof["traffic_source"] = os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+            roof["traffic_measured_steps_per_launch"] = traffic_steps
+    head = stress or {"value": None, "wall_s": float("nan"), "workload": None, "replicas": None,
+                      "plain": {"value": None}, "speedup_vs_plain": None}
     out = {
-        "metric": METRIC, "value": value, "unit": "pose evaluations/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": c2["wall"] * 1e3 / args.steps, "higher_is_better": True,
+        "metric": METRIC, "value": head["value"], "unit": "pose evaluations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["wall_s"] * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": c2["workload"], "replicas": c2["n_rep"],
-                   "form": "batched (packed replicas, autotuned shape, two workgroup groups on two streams, hipGraph chunks)"},
-        "plain": plain,
-        "speedup_vs_plain": value / plain["value"] if (value and plain["value"]) else None,
-        "roofline": c2["roofline"],
+        "config": {"workload": head["workload"], "replicas": head["replicas"],
+                   "form": "batched (persistent fused kernel over a cost-ordered work queue, autotuned shape, "
+                           "hipGraph chunks)"},
+        "plain": head["plain"],
+        "speedup_vs_plain": head["speedup_vs_plain"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "c2": c2,
         "screen": screen,
         "serve": serve,
-        "stress": stress,
-        "cpu_baseline": cpu,
     }
     if serve:
         out["p50_ms"] = {str(r["rate_per_s"]): r["p50_ms"] for r in serve["rates"]}

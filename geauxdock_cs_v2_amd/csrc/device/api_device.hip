@@ -141,9 +141,19 @@ JobRes *find_job(gd_ctx *ctx, int id)
 // src/frontend/frontend.cpp:267-299): replica r = n_tmp n_lig p + n_lig t + l
 // belongs to ligand conformation l and protein conformation p, and every
 // (l, p) ladder holds each temperature exactly once (the exchange kernel
-// finds a ladder's replicas through their temperature labels).
+// finds a ladder's replicas through their temperature labels).  idx_rep keys
+// the replica's random stream and indexes explicit stream words
+// ([idx_rep][step][7]), so it must be a permutation of [0, n_rep): a value
+// outside would read beyond the words, a duplicate would share a stream.
 int check_replica_layout(const gd_ReplicaMC *rep, int n_lig, int n_prt, int n_tmp)
 {
+    const size_t n_rep = (size_t)n_lig * n_prt * n_tmp;
+    std::vector<char> rep_seen(n_rep, 0);
+    for (size_t r = 0; r < n_rep; ++r) {
+        const int k = rep[r].idx_rep;
+        if (k < 0 || (size_t)k >= n_rep) return fail(GD_ERR_ARG, "replica idx_rep out of range");
+        if (rep_seen[k]++) return fail(GD_ERR_ARG, "replica idx_rep values are not a permutation");
+    }
     std::vector<int> seen(n_tmp);
     for (int p = 0; p < n_prt; ++p)
         for (int l = 0; l < n_lig; ++l) {
@@ -202,6 +212,9 @@ void gd_destroy(gd_ctx *ctx)
     ctx->pose_move.release();
     ctx->pose_out.release();
     ctx->counter.release();
+    ctx->queue.release();
+    ctx->ladders.release();
+    ctx->bases.release();
     for (hipStream_t &g : ctx->group_stream)
         if (g) (void)hipStreamDestroy(g);
     (void)hipStreamDestroy(ctx->stream);

@@ -1,10 +1,14 @@
 // Batched form: every replica of several jobs against one resident target in
-// fused launches per chunk of steps, with replicas packed R per workgroup so
-// their ligand atoms fill the 64-lane waves, the protein points split across
-// `slices` waves, and workgroup shapes picked by a timing autotuner.  The
-// tasks run as two workgroup groups, one launch each per chunk on streams of
-// their own; a group's chunk sequence (Monte Carlo chunk, exchange round,
-// step advance) can be captured once in a hipGraph and replayed.
+// one persistent launch per chunk of steps.  Replicas are packed R per
+// workgroup so their ligand atoms fill the 64-lane waves, the protein points
+// are split across `slices` waves, and workgroup shapes and energy layouts
+// are picked by a timing autotuner (per job, or per ligand-size bucket when
+// several jobs share a launch).  The launch's grid is the number of
+// workgroups resident at once; they walk a cost-ordered work queue of
+// (task, step slice) items (see mc_kernel), so the last round of a chunk
+// spreads over the whole GPU.  A chunk (queue reset, Monte Carlo launch,
+// exchange round over every job's ladders, step advance) is captured once in
+// a hipGraph and replayed.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -35,7 +39,33 @@ __global__ void advance_kernel(int *step_base, int dstep, int *round_base)
 }
 
 std::mutex g_tune_mu;
+// single job: (N_L, N_P, n_rep, n_prt) -> shape
 std::map<std::tuple<int, int, int, int>, Shape> g_tune_cache;
+// several jobs: (N_L bucket, N_P, waves) -> shape of the bucket's jobs
+std::map<std::tuple<int, int, int>, Shape> g_bucket_cache;
+// several jobs: (N_P, bucket signature) -> waves per workgroup
+std::map<std::pair<int, std::vector<std::pair<int, int>>>, int> g_waves_cache;
+
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() = default;
+    Event(const Event &) = delete;
+    ~Event()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// graph of one chunk, released with the scope
+struct Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~Graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
 
 } // namespace
 
@@ -49,6 +79,7 @@ static Shape packed_shape(int natom, int W, int SW, int max_rep_block)
     s.slot_waves = SW;
     s.slices = W / SW;
     s.nrep = std::max(1, std::min({(64 * SW) / natom, max_rep_block, 64}));
+    s.layout = 0;
     return s;
 }
 
@@ -90,7 +121,7 @@ static void build_unit_tasks(const JobRes &j, int job_slot, const Shape &sh, int
 {
     const int block = j.n_tmp * j.n_lig; // replicas of one protein conformation are contiguous
     for (int r = 0; r < block; r += sh.nrep) {
-        DevTask t;
+        DevTask t{};
         t.job = job_slot;
         t.rep0 = p * block + r;
         t.nrep = std::min(sh.nrep, block - r);
@@ -99,6 +130,7 @@ static void build_unit_tasks(const JobRes &j, int job_slot, const Shape &sh, int
         t.s2max = s2max;
         t.slices = sh.slices;
         t.slot_waves = sh.slot_waves;
+        t.layout = sh.layout;
         out.push_back(t);
     }
 }
@@ -108,131 +140,166 @@ void build_tasks(const JobRes &j, int job_slot, const Shape &sh, int s1, int s2m
     for (int p = 0; p < j.n_prt; ++p) build_unit_tasks(j, job_slot, sh, p, s1, s2max, out);
 }
 
-namespace {
-
-// Workgroup groups.  A launch ends when its slowest workgroup does, so when
-// its task count is not a multiple of the workgroups resident at once the
-// last round leaves CUs idle until the next launch (C2: 880 tasks over 512
-// resident workgroups idles 14% of the GPU).  The batch is therefore split
-// into groups of whole (job, protein conformation) units, each launched on
-// a stream of its own with its own task table, step counters and graph:
-// one group's tail overlaps the other's next launch.  Every temperature
-// ladder lies within one unit, so the exchange rounds split with the
-// groups, and the results do not depend on the split.
-struct Group {
-    struct Range {
-        int job, p0, p1; // protein conformations p0 .. p1 - 1 of batch job `job`
-    };
-    std::vector<Range> ranges;
-    std::vector<DevTask> tasks, tail;
-    DevBuf tasks_dev, bases; // bases: step and exchange-round counters advanced in the graph
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    Group() = default;
-    Group(const Group &) = delete;
-    Group(Group &&o) noexcept
-        : ranges(std::move(o.ranges)), tasks(std::move(o.tasks)), tail(std::move(o.tail)),
-          tasks_dev(std::move(o.tasks_dev)), bases(std::move(o.bases)), graph(o.graph), exec(o.exec)
-    {
-        o.graph = nullptr;
-        o.exec = nullptr;
-    }
-    ~Group()
-    {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-    }
-    int *step_base() const { return bases.as<int>(); }
-    int *round_base() const { return bases.as<int>() + 1; }
-};
-
-// waits for the group streams when it goes out of scope: declared after the
-// groups, it runs first, so no graph or buffer is released while in use
-struct GroupsIdle {
-    gd_ctx *ctx;
-    ~GroupsIdle()
-    {
-        for (hipStream_t g : ctx->group_stream)
-            if (g) (void)hipStreamSynchronize(g);
-    }
-};
-
-struct Event {
-    hipEvent_t e = nullptr;
-    Event() = default;
-    Event(const Event &) = delete;
-    ~Event()
-    {
-        if (e) (void)hipEventDestroy(e);
-    }
-};
-
-// The units (job i, conformation p) in order, cut into at most G runs of
-// about equal task counts (a unit goes to the run holding its midpoint).
-std::vector<Group> make_groups(const std::vector<JobRes *> &js, const std::vector<Shape> &shapes, int G)
+int resident_slots(gd_ctx *ctx, int waves, int max_rep)
 {
-    long total = 0;
-    for (size_t i = 0; i < js.size(); ++i)
-        total += (long)js[i]->n_prt * ((js[i]->n_tmp * js[i]->n_lig + shapes[i].nrep - 1) / shapes[i].nrep);
-    std::vector<Group> gr(std::max(1, G));
-    long before = 0;
-    for (size_t i = 0; i < js.size(); ++i) {
-        const long c = (js[i]->n_tmp * js[i]->n_lig + shapes[i].nrep - 1) / shapes[i].nrep;
-        for (int p = 0; p < js[i]->n_prt; ++p) {
-            const int g = std::min((int)gr.size() - 1, (int)(((double)before + 0.5 * c) * gr.size() / total));
-            auto &rs = gr[g].ranges;
-            if (!rs.empty() && rs.back().job == (int)i && rs.back().p1 == p)
-                rs.back().p1 = p + 1;
-            else
-                rs.push_back({(int)i, p, p + 1});
-            before += c;
-        }
-    }
-    std::vector<Group> out;
-    for (Group &g : gr)
-        if (!g.ranges.empty()) out.push_back(std::move(g));
-    return out;
+    return std::max(1, ctx->cu_count) * mc_blocks_per_cu(waves, max_rep);
 }
 
-// a group's tasks for s2max steps from the batch jobs' own step counts,
-// in decreasing cost order so heavy workgroups start first
-void group_tasks(const Group &g, const std::vector<JobRes *> &js, const std::vector<Shape> &shapes,
-                 const std::vector<int> &s1_of, int s2max, std::vector<DevTask> &out)
+// Step slices per task.  With ntask tasks of equal cost on `slots` resident
+// workgroups a chunk takes ceil(ntask K / slots) / K task durations for K
+// slices per task (ntask / slots at best); each slice adds a hand-over (the
+// replica states through L2 and an acquire), counted as 0.5% of a slice.
+int choose_slices(int ntask, int slots, int steps)
+{
+    if (ntask <= slots || steps <= 1) return 1;
+    int best = 1;
+    double best_cost = 1e30;
+    for (int k = 1; k <= std::min(8, steps); ++k) {
+        const double rounds = std::ceil((double)ntask * k / slots) / k;
+        const double cost = rounds * (1.0 + 0.005 * (k - 1));
+        if (cost < best_cost - 1e-12) {
+            best_cost = cost;
+            best = k;
+        }
+    }
+    return best;
+}
+
+namespace {
+
+// all tasks of the batch for s2max steps from each job's own step count, in
+// decreasing cost order (pair evaluations per step) so heavy workgroups
+// start first
+void batch_tasks(const std::vector<JobRes *> &js, const std::vector<Shape> &shapes, const std::vector<int> &s1_of,
+                 int s2max, std::vector<DevTask> &out)
 {
     out.clear();
-    for (const Group::Range &r : g.ranges)
-        for (int p = r.p0; p < r.p1; ++p)
-            build_unit_tasks(*js[r.job], r.job, shapes[r.job], p, s1_of[r.job], s2max, out);
+    for (size_t i = 0; i < js.size(); ++i) build_tasks(*js[i], (int)i, shapes[i], s1_of[i], s2max, out);
     std::stable_sort(out.begin(), out.end(), [&](const DevTask &x, const DevTask &y) {
         return (long)x.nrep * js[x.job]->natom > (long)y.nrep * js[y.job]->natom;
     });
 }
 
-// order the group streams after the work on s (fork), or s after the group
-// streams (join)
-int fork_groups(gd_ctx *ctx, hipStream_t s, size_t n, hipEvent_t ev)
+int max_rep_of(const std::vector<Shape> &shapes)
 {
-    GD_HIP(hipEventRecord(ev, s));
-    for (size_t g = 0; g < n; ++g) GD_HIP(hipStreamWaitEvent(ctx->group_stream[g], ev, 0));
+    int m = 1;
+    for (const Shape &s : shapes) m = std::max(m, s.nrep);
+    return m;
+}
+
+// the context's work queue, with room for ntask tasks
+int ensure_queue(gd_ctx *ctx, int ntask, DevQueue &q)
+{
+    const size_t need = sizeof(int) * ((size_t)ntask + 1);
+    if (ctx->queue.bytes < need) GD_HIP(ctx->queue.alloc(need));
+    q.counter = ctx->queue.as<int>();
+    q.done = ctx->queue.as<int>() + 1;
+    q.ntask = ntask;
+    q.nslice = 1;
     return GD_OK;
 }
 
-int join_groups(gd_ctx *ctx, hipStream_t s, size_t n, Event *ev)
+// One persistent launch of `tasks_dev` (ntask tasks, shape waves/max_rep,
+// slices chosen for s2max steps): queue reset, Monte Carlo kernel.
+int enqueue_mc(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const DevTask *tasks_dev, int ntask,
+               int s2max, int waves, int max_rep, bool comp, bool records, hipStream_t s, const int *step_base)
 {
-    for (size_t g = 0; g < n; ++g) {
-        GD_HIP(hipEventRecord(ev[g].e, ctx->group_stream[g]));
-        GD_HIP(hipStreamWaitEvent(s, ev[g].e, 0));
-    }
+    DevQueue q;
+    if (int rc = ensure_queue(ctx, ntask, q)) return rc;
+    const int slots = resident_slots(ctx, waves, max_rep);
+    q.nslice = choose_slices(ntask, slots, s2max);
+    GD_HIP(launch_queue_reset(q, s));
+    GD_HIP(launch_mc_queue(t.dt, t.sc, jobs_dev, tasks_dev, q, slots, waves, max_rep, comp, records, s, step_base));
     return GD_OK;
+}
+
+// Time `launches` consecutive persistent launches of `steps` steps of the
+// batch with the given shapes, after one untimed launch (code object,
+// caches); the replica states and McLog counters are restored afterwards.
+int time_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js, const DevJob *jobs_dev,
+                const std::vector<Shape> &shapes, int steps, int launches, bool warm, bool comp, float &ms)
+{
+    hipStream_t s = ctx->stream;
+    std::vector<int> s1_of(js.size());
+    for (size_t i = 0; i < js.size(); ++i) s1_of[i] = js[i]->steps_done;
+    std::vector<DevTask> tasks;
+    batch_tasks(js, shapes, s1_of, steps, tasks);
+    const int W = shapes[0].waves;
+    const int max_rep = max_rep_of(shapes);
+    DevBuf tasks_dev;
+    GD_HIP(upload(tasks_dev, tasks.data(), sizeof(DevTask) * tasks.size(), s));
+    Event e0, e1;
+    GD_HIP(hipEventCreate(&e0.e));
+    GD_HIP(hipEventCreate(&e1.e));
+    if (warm)
+        if (int rc = enqueue_mc(ctx, t, jobs_dev, tasks_dev.as<DevTask>(), (int)tasks.size(), steps, W, max_rep,
+                                comp, false, s, nullptr))
+            return rc;
+    GD_HIP(hipEventRecord(e0.e, s));
+    for (int k = 0; k < launches; ++k)
+        if (int rc = enqueue_mc(ctx, t, jobs_dev, tasks_dev.as<DevTask>(), (int)tasks.size(), steps, W, max_rep,
+                                comp, false, s, nullptr))
+            return rc;
+    GD_HIP(hipEventRecord(e1.e, s));
+    GD_HIP(hipEventSynchronize(e1.e));
+    GD_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+    return GD_OK;
+}
+
+// the replica states and McLog counters of the batch's jobs, saved on the
+// device and put back when the trials are over
+struct StateBackup {
+    std::vector<DevBuf> rep, log;
+    int save(gd_ctx *ctx, const std::vector<JobRes *> &js)
+    {
+        rep.resize(js.size());
+        log.resize(js.size());
+        for (size_t i = 0; i < js.size(); ++i) {
+            GD_HIP(rep[i].alloc(js[i]->rep.bytes));
+            GD_HIP(log[i].alloc(js[i]->mclog.bytes));
+            GD_HIP(hipMemcpyAsync(rep[i].p, js[i]->rep.p, js[i]->rep.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            GD_HIP(hipMemcpyAsync(log[i].p, js[i]->mclog.p, js[i]->mclog.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        return GD_OK;
+    }
+    int restore(gd_ctx *ctx, const std::vector<JobRes *> &js)
+    {
+        for (size_t i = 0; i < js.size(); ++i) {
+            GD_HIP(hipMemcpyAsync(js[i]->rep.p, rep[i].p, js[i]->rep.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            GD_HIP(hipMemcpyAsync(js[i]->mclog.p, log[i].p, js[i]->mclog.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        GD_HIP(hipStreamSynchronize(ctx->stream));
+        return GD_OK;
+    }
+};
+
+// trial length: a few steps per launch, one step once a step holds more than
+// ~1e11 pair evaluations (a few 100 ms); never more than the run's own
+// `cap` steps, which the explicit stream words (if any) were checked to cover
+int trial_steps(const std::vector<JobRes *> &js, const TargetRes &t, int cap)
+{
+    double pairs = 0.0;
+    for (const JobRes *j : js) pairs += (double)j->n_rep * j->natom * t.npoint;
+    return std::max(1, std::min(cap, pairs > 1e11 ? 1 : 4));
+}
+
+// ligand-size bucket of the multi-job autotuner: jobs whose atom counts fall
+// in one bucket share a packing
+int natom_bucket(int natom)
+{
+    static const int edges[] = {16, 24, 32, 40, 48, 64, 80, 96, 128, 192, 256, 384, 512, 768, 1024};
+    for (int e : edges)
+        if (natom <= e) return e;
+    return 1 << 30;
 }
 
 } // namespace
 
-// Autotuner: time a short run of consecutive launches of every candidate
-// workgroup shape on the job's own data (state saved and restored around the
-// trials), launched in workgroup groups as the batched runs are, and keep the
-// fastest; results are cached per (N_L, N_P, n_rep, n_prt).
-int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool compensated, Shape &out)
+// Autotuner (one job): time every candidate workgroup shape on the job's own
+// data (state saved and restored around the trials) as persistent launches,
+// then the KDE/MCS layouts on the fastest shape; the result is cached per
+// (N_L, N_P, n_rep, n_prt).
+int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, bool compensated, int max_steps, Shape &out)
 {
     const auto key = std::make_tuple(j.natom, t.npoint, j.n_rep, j.n_prt);
     {
@@ -253,73 +320,173 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, int s1, bool comp
             if (W % SW || 64 * SW < j.natom) continue;
             const Shape s = packed_shape(j.natom, W, SW, block);
             bool dup = false;
-            for (const Shape &c : cands)
-                dup |= c.waves == s.waves && c.nrep == s.nrep && c.slices == s.slices;
+            for (const Shape &c : cands) dup |= c.waves == s.waves && c.nrep == s.nrep && c.slices == s.slices;
             if (!dup) cands.push_back(s);
         }
     if (cands.empty()) return fail(GD_ERR_ARG, "ligand too large for a 16-wave workgroup");
-    if (int rc = ensure_group_streams(ctx)) return rc;
-    hipStream_t s = ctx->stream;
-    // the trials change the replica states and the acceptance counters
-    DevBuf backup, backup_log;
-    GD_HIP(backup.alloc(j.rep.bytes));
-    GD_HIP(backup_log.alloc(j.mclog.bytes));
-    GD_HIP(hipMemcpyAsync(backup.p, j.rep.p, j.rep.bytes, hipMemcpyDeviceToDevice, s));
-    GD_HIP(hipMemcpyAsync(backup_log.p, j.mclog.p, j.mclog.bytes, hipMemcpyDeviceToDevice, s));
-    Event e0, e1, ef, ej[gd_ctx::kGroups];
-    for (Event *e : {&e0, &e1, &ef}) GD_HIP(hipEventCreate(&e->e));
-    for (Event &e : ej) GD_HIP(hipEventCreateWithFlags(&e.e, hipEventDisableTiming));
-    // Each candidate is timed as the real chunk sequence runs: three
-    // consecutive 2-step launches per group, alternating over the group
-    // streams, so that one group's tail overlaps the other's next launch (a
-    // single launch per group times the tails the sequence hides, and ranks
-    // C2's best shapes within noise of each other; three launches rank them
-    // as full 10,000-step runs do).  Once a step holds more than ~1e11 pair
-    // evaluations (a few 100 ms) one 1-step launch per group, no warm-up.
-    const double pairs = (double)j.n_rep * j.natom * t.npoint;
-    const bool large = pairs > 1e11;
-    const int trial = large ? 1 : 2;
-    const int trial_chunks = large ? 1 : 3;
+    std::vector<JobRes *> one{&j};
+    const DevJob *jobs_dev = j.self.as<DevJob>();
+    const int steps = trial_steps(one, t, max_steps);
+    const bool large = (double)j.n_rep * j.natom * t.npoint > 1e11;
+    StateBackup bk;
+    if (int rc = bk.save(ctx, one)) return rc;
     float best_ms = 1e30f;
     Shape best = cands[0];
-    std::vector<JobRes *> one{&j};
-    const std::vector<int> s1_of{s1};
     for (const Shape &c : cands) {
-        const std::vector<Shape> shapes{c};
-        std::vector<Group> gr = make_groups(one, shapes, gd_ctx::kGroups);
-        GroupsIdle idle{ctx};
-        for (Group &g : gr) {
-            group_tasks(g, one, shapes, s1_of, trial, g.tasks);
-            GD_HIP(upload(g.tasks_dev, g.tasks.data(), sizeof(DevTask) * g.tasks.size(), s));
-        }
-        // warm the code object and caches with one launch per group, then time
-        for (int rep = large ? 1 : 0; rep < 2; ++rep) {
-            GD_HIP(hipEventRecord(e0.e, s));
-            if (int rc = fork_groups(ctx, s, gr.size(), ef.e)) return rc;
-            for (int k = 0; k < (rep ? trial_chunks : 1); ++k)
-                for (size_t g = 0; g < gr.size(); ++g)
-                    GD_HIP(launch_mc(t.dt, t.sc, j.self.as<DevJob>(), gr[g].tasks_dev.as<DevTask>(),
-                                     (int)gr[g].tasks.size(), c.waves, c.nrep, compensated, false,
-                                     ctx->group_stream[g], nullptr));
-            if (int rc = join_groups(ctx, s, gr.size(), ej)) return rc;
-            GD_HIP(hipEventRecord(e1.e, s));
-            GD_HIP(hipMemcpyAsync(j.rep.p, backup.p, j.rep.bytes, hipMemcpyDeviceToDevice, s));
-            GD_HIP(hipMemcpyAsync(j.mclog.p, backup_log.p, j.mclog.bytes, hipMemcpyDeviceToDevice, s));
-            GD_HIP(hipEventSynchronize(e1.e));
-        }
         float ms = 0.f;
-        GD_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+        if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, large ? 1 : 2, !large, compensated, ms)) return rc;
+        if (int rc = bk.restore(ctx, one)) return rc;
         if (ms < best_ms) {
             best_ms = ms;
             best = c;
         }
     }
-    GD_HIP(hipStreamSynchronize(s));
+    // energy layouts on the chosen shape (a single-slice shape has one layout)
+    if (best.slices > 1)
+        for (int layout : {(int)LAYOUT_KDE_SLICE0, (int)LAYOUT_MCS_SLICE0, LAYOUT_KDE_SLICE0 | LAYOUT_MCS_SLICE0}) {
+            Shape c = best;
+            c.layout = layout;
+            float ms = 0.f;
+            if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, large ? 1 : 2, !large, compensated, ms))
+                return rc;
+            if (int rc = bk.restore(ctx, one)) return rc;
+            if (ms < best_ms * 0.995f) { // a layout change must win by more than the noise
+                best_ms = ms;
+                best = c;
+            }
+        }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = best;
     }
     out = best;
+    return GD_OK;
+}
+
+// Autotuner (several jobs in one launch): the wave count W is shared by the
+// launch, the packing per ligand-size bucket.  First W from {8, 12, 16} with
+// the packing heuristic, then, bucket by bucket in decreasing share of the
+// work (buckets above 5% of it), every slot-wave count that divides W, then
+// the energy layouts for all jobs; every trial times the whole batch.
+// Results are cached per (N_P, bucket signature) for W and per (bucket, N_P,
+// W) for the packings, which later batches reuse.
+int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js, const DevJob *jobs_dev,
+                   bool compensated, int max_steps, std::vector<Shape> &shapes)
+{
+    const size_t n = js.size();
+    std::map<int, double> work; // bucket -> pair evaluations per step
+    std::map<int, int> count;
+    std::vector<int> bucket(n);
+    double total = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        bucket[i] = natom_bucket(js[i]->natom);
+        const double w = (double)js[i]->n_rep * js[i]->natom * t.npoint;
+        work[bucket[i]] += w;
+        count[bucket[i]] += 1;
+        total += w;
+    }
+    std::vector<std::pair<int, int>> sig(count.begin(), count.end());
+    int max_natom = 0;
+    for (const JobRes *j : js) max_natom = std::max(max_natom, j->natom);
+    const auto heuristic_all = [&](int W, std::vector<Shape> &sh) {
+        const int share = std::max(1, resident_slots(ctx, W, 8) / (int)n);
+        sh.resize(n);
+        for (size_t i = 0; i < n; ++i)
+            sh[i] = heuristic_shape(js[i]->natom, W, js[i]->n_tmp * js[i]->n_lig, js[i]->n_prt, share);
+    };
+    // everything cached: no trials
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto wit = g_waves_cache.find({t.npoint, sig});
+        if (wit != g_waves_cache.end()) {
+            const int W = wit->second;
+            std::vector<Shape> sh;
+            heuristic_all(W, sh);
+            bool all = true;
+            for (size_t i = 0; i < n && all; ++i) {
+                auto b = g_bucket_cache.find(std::make_tuple(bucket[i], t.npoint, W));
+                if (b == g_bucket_cache.end()) {
+                    all = false;
+                } else {
+                    sh[i] = packed_shape(js[i]->natom, W, b->second.slot_waves, js[i]->n_tmp * js[i]->n_lig);
+                    sh[i].layout = b->second.layout;
+                }
+            }
+            if (all) {
+                shapes = sh;
+                return GD_OK;
+            }
+        }
+    }
+    const int steps = trial_steps(js, t, max_steps);
+    StateBackup bk;
+    if (int rc = bk.save(ctx, js)) return rc;
+    auto trial = [&](const std::vector<Shape> &sh, float &ms) -> int {
+        float a = 0.f;
+        if (int rc = time_shapes(ctx, t, js, jobs_dev, sh, steps, 1, true, compensated, a)) return rc;
+        if (int rc = bk.restore(ctx, js)) return rc;
+        ms = a;
+        return GD_OK;
+    };
+    float best_ms = 1e30f;
+    std::vector<Shape> best;
+    for (int W : {8, 12, 16}) {
+        if (max_natom > 64 * W) continue;
+        std::vector<Shape> sh;
+        heuristic_all(W, sh);
+        float ms = 0.f;
+        if (int rc = trial(sh, ms)) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = sh;
+        }
+    }
+    if (best.empty()) return fail(GD_ERR_ARG, "ligand too large for a 16-wave workgroup");
+    const int W = best[0].waves;
+    std::vector<std::pair<double, int>> order;
+    for (const auto &kv : work) order.emplace_back(-kv.second, kv.first);
+    std::sort(order.begin(), order.end());
+    for (const auto &o : order) {
+        const int b = o.second;
+        if (-o.first < 0.05 * total) continue;
+        int bmax = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (bucket[i] == b) bmax = std::max(bmax, js[i]->natom);
+        for (int SW = 1; SW <= W; ++SW) {
+            if (W % SW || 64 * SW < bmax) continue;
+            std::vector<Shape> sh = best;
+            bool changed = false;
+            for (size_t i = 0; i < n; ++i)
+                if (bucket[i] == b) {
+                    changed |= sh[i].slot_waves != SW;
+                    sh[i] = packed_shape(js[i]->natom, W, SW, js[i]->n_tmp * js[i]->n_lig);
+                    sh[i].layout = best[i].layout;
+                }
+            if (!changed) continue;
+            float ms = 0.f;
+            if (int rc = trial(sh, ms)) return rc;
+            if (ms < best_ms * 0.995f) {
+                best_ms = ms;
+                best = sh;
+            }
+        }
+    }
+    for (int layout : {(int)LAYOUT_KDE_SLICE0, (int)LAYOUT_MCS_SLICE0, LAYOUT_KDE_SLICE0 | LAYOUT_MCS_SLICE0}) {
+        std::vector<Shape> sh = best;
+        for (Shape &s : sh) s.layout = s.slices > 1 ? layout : 0;
+        float ms = 0.f;
+        if (int rc = trial(sh, ms)) return rc;
+        if (ms < best_ms * 0.995f) {
+            best_ms = ms;
+            best = sh;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_waves_cache[{t.npoint, sig}] = W;
+        for (size_t i = 0; i < n; ++i) g_bucket_cache[std::make_tuple(bucket[i], t.npoint, W)] = best[i];
+    }
+    shapes = best;
     return GD_OK;
 }
 
@@ -363,9 +530,25 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
     const int chunk = opts->exchange_interval > 0 ? opts->exchange_interval : std::min(opts->steps, GD_STEPS_PER_DUMP);
     const bool records = opts->keep_records != 0;
     hipStream_t s = ctx->stream;
+    std::vector<int> s1_of(n_jobs), round_of(n_jobs);
+    for (int i = 0; i < n_jobs; ++i) {
+        s1_of[i] = js[i]->steps_done;
+        round_of[i] = js[i]->exchange_rounds;
+        if (int rc = check_stream_range(*js[i], (long)s1_of[i] + opts->steps)) return rc;
+    }
 
-    // workgroup shapes: explicit, autotuned (single job) or the packing
-    // heuristic under one common wave count (several jobs)
+    // device job table for this batch
+    std::vector<DevJob> table(n_jobs);
+    for (int i = 0; i < n_jobs; ++i) {
+        if (records)
+            if (int rc = ensure_records(ctx, *js[i], chunk)) return rc;
+        table[i] = js[i]->hj;
+    }
+    DevBuf jobs_dev;
+    GD_HIP(upload(jobs_dev, table.data(), sizeof(DevJob) * n_jobs, s));
+
+    // workgroup shapes: explicit, or autotuned (per job, or per ligand-size
+    // bucket for several jobs)
     int W = opts->waves_per_group;
     std::vector<Shape> shapes(n_jobs);
     if (opts->slots_per_wave > 0 || opts->point_slices > 0) {
@@ -379,101 +562,86 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
             if (opts->slots_per_wave > 0)
                 shapes[i].nrep = std::max(1, std::min(shapes[i].nrep, opts->slots_per_wave * SW / js[i]->natom));
         }
-    } else if (n_jobs == 1 && W <= 0) {
-        if (int rc = autotune_shape(ctx, t, *js[0], 0, comp, shapes[0])) return rc;
-        W = shapes[0].waves;
-    } else {
-        if (W <= 0) W = 12;
+    } else if (W > 0) {
         if (W > 16) return fail(GD_ERR_ARG, "waves_per_group must be at most 16");
-        // each job's share of the workgroups resident at once (two per CU)
-        const int slots = std::max(1, 2 * std::max(ctx->cu_count, 1) / n_jobs);
+        const int share = std::max(1, resident_slots(ctx, W, 8) / n_jobs);
         for (int i = 0; i < n_jobs; ++i) {
             if (js[i]->natom > 64 * W) return fail(GD_ERR_ARG, "ligand too large for the workgroup");
-            shapes[i] = heuristic_shape(js[i]->natom, W, js[i]->n_tmp * js[i]->n_lig, js[i]->n_prt, slots);
+            shapes[i] = heuristic_shape(js[i]->natom, W, js[i]->n_tmp * js[i]->n_lig, js[i]->n_prt, share);
         }
+    } else if (n_jobs == 1) {
+        if (int rc = autotune_shape(ctx, t, *js[0], comp, opts->steps, shapes[0])) return rc;
+        W = shapes[0].waves;
+    } else {
+        if (int rc = autotune_batch(ctx, t, js, jobs_dev.as<DevJob>(), comp, opts->steps, shapes)) return rc;
+        W = shapes[0].waves;
     }
-    int max_rep = 1;
-    for (const Shape &sh : shapes) max_rep = std::max(max_rep, sh.nrep);
+    const int max_rep = max_rep_of(shapes);
     for (int i = 0; i < n_jobs; ++i) js[i]->last_shape = shapes[i];
 
-    // device job table for this batch
-    std::vector<DevJob> table(n_jobs);
-    for (int i = 0; i < n_jobs; ++i) {
-        if (records)
-            if (int rc = ensure_records(ctx, *js[i], chunk)) return rc;
-        table[i] = js[i]->hj;
-    }
-    if (int rc = ensure_group_streams(ctx)) return rc;
-    DevBuf jobs_dev;
-    GD_HIP(upload(jobs_dev, table.data(), sizeof(DevJob) * n_jobs, s));
-    std::vector<int> s1_of(n_jobs), round_of(n_jobs);
-    for (int i = 0; i < n_jobs; ++i) {
-        s1_of[i] = js[i]->steps_done;
-        round_of[i] = js[i]->exchange_rounds;
-        if (int rc = check_stream_range(*js[i], (long)s1_of[i] + opts->steps)) return rc;
-    }
-    std::vector<Group> gr = make_groups(js, shapes, gd_ctx::kGroups);
-    GroupsIdle idle{ctx};
+    // tasks of a full chunk, and the exchange ladders of every job
+    std::vector<DevTask> tasks;
+    batch_tasks(js, shapes, s1_of, chunk, tasks);
+    const int ntask = (int)tasks.size();
+    DevBuf tasks_dev;
+    GD_HIP(upload(tasks_dev, tasks.data(), sizeof(DevTask) * ntask, s));
+    std::vector<int4> lad;
+    if (opts->exchange_interval > 0)
+        for (int i = 0; i < n_jobs; ++i)
+            if (js[i]->n_tmp > 1)
+                for (int L = 0; L < js[i]->n_lig * js[i]->n_prt; ++L) lad.push_back(make_int4(i, L, round_of[i], 0));
+    if (!lad.empty()) GD_HIP(upload(ctx->ladders, lad.data(), sizeof(int4) * lad.size(), s));
+    if (ctx->bases.bytes < 2 * sizeof(int)) GD_HIP(ctx->bases.alloc(2 * sizeof(int)));
+    int *step_base = ctx->bases.as<int>(), *round_base = ctx->bases.as<int>() + 1;
     const int zero2[2] = {0, 0};
-    for (Group &g : gr) {
-        group_tasks(g, js, shapes, s1_of, chunk, g.tasks);
-        GD_HIP(upload(g.tasks_dev, g.tasks.data(), sizeof(DevTask) * g.tasks.size(), s));
-        GD_HIP(g.bases.alloc(2 * sizeof(int)));
-        GD_HIP(hipMemcpyAsync(g.bases.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
-    }
+    GD_HIP(hipMemcpyAsync(ctx->bases.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+    DevQueue q0;
+    if (int rc = ensure_queue(ctx, ntask, q0)) return rc; // sized before any capture
 
-    // one chunk of a group: Monte Carlo launch, exchange round over the
-    // group's ladders, step advance (no memset, so it can be captured)
-    auto enqueue_period = [&](Group &g, hipStream_t gs, int s2) -> int {
-        GD_HIP(launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), g.tasks_dev.as<DevTask>(), (int)g.tasks.size(), W, max_rep,
-                         comp, records, gs, g.step_base()));
-        if (opts->exchange_interval > 0)
-            for (const Group::Range &r : g.ranges) {
-                const JobRes &j = *js[r.job];
-                GD_HIP(launch_exchange(jobs_dev.as<DevJob>() + r.job, r.p0 * j.n_lig, (r.p1 - r.p0) * j.n_lig, j.n_lig,
-                                       j.n_tmp, round_of[r.job], nullptr, gs, g.round_base()));
-            }
-        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, gs, g.step_base(), s2, g.round_base());
+    // one chunk: queue reset, persistent Monte Carlo launch, exchange round
+    // over every ladder, step advance (no memset, so it can be captured)
+    auto enqueue_period = [&]() -> int {
+        if (int rc = enqueue_mc(ctx, t, jobs_dev.as<DevJob>(), tasks_dev.as<DevTask>(), ntask, chunk, W, max_rep, comp,
+                                records, s, step_base))
+            return rc;
+        if (!lad.empty())
+            GD_HIP(launch_exchange_list(jobs_dev.as<DevJob>(), ctx->ladders.as<int4>(), (int)lad.size(), s, round_base));
+        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, step_base, chunk, round_base);
         GD_HIP(hipGetLastError());
         return GD_OK;
     };
 
     const int n_full = opts->steps / chunk;
     const int rest = opts->steps - n_full * chunk;
-    if (n_full > 0 && opts->use_graph)
-        for (size_t k = 0; k < gr.size(); ++k) {
-            hipStream_t gs = ctx->group_stream[k];
-            GD_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue_period(gr[k], gs, chunk);
-            const hipError_t ce = hipStreamEndCapture(gs, &gr[k].graph);
-            if (rc) return rc;
-            GD_HIP(ce);
-            GD_HIP(hipGraphInstantiate(&gr[k].exec, gr[k].graph, nullptr, nullptr, 0));
-        }
-    Event e0, e1, ef, ej[gd_ctx::kGroups];
-    for (Event *e : {&e0, &e1, &ef}) GD_HIP(hipEventCreate(&e->e));
-    for (Event &e : ej) GD_HIP(hipEventCreateWithFlags(&e.e, hipEventDisableTiming));
+    Graph g;
+    if (n_full > 0 && opts->use_graph) {
+        GD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_period();
+        const hipError_t ce = hipStreamEndCapture(s, &g.graph);
+        if (rc) return rc;
+        GD_HIP(ce);
+        GD_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    }
+    std::vector<DevTask> tail;
+    DevBuf tail_dev;
+    if (rest > 0) {
+        batch_tasks(js, shapes, s1_of, rest, tail);
+        GD_HIP(upload(tail_dev, tail.data(), sizeof(DevTask) * tail.size(), s));
+    }
+    Event e0, e1;
+    GD_HIP(hipEventCreate(&e0.e));
+    GD_HIP(hipEventCreate(&e1.e));
     GD_HIP(hipEventRecord(e0.e, s));
-    if (int rc = fork_groups(ctx, s, gr.size(), ef.e)) return rc;
-    for (int c = 0; c < n_full; ++c)
-        for (size_t k = 0; k < gr.size(); ++k) {
-            if (opts->use_graph)
-                GD_HIP(hipGraphLaunch(gr[k].exec, ctx->group_stream[k]));
-            else if (int rc = enqueue_period(gr[k], ctx->group_stream[k], chunk))
-                return rc;
-        }
+    for (int c = 0; c < n_full; ++c) {
+        if (opts->use_graph)
+            GD_HIP(hipGraphLaunch(g.exec, s));
+        else if (int rc = enqueue_period())
+            return rc;
+    }
     if (rest > 0)
-        for (size_t k = 0; k < gr.size(); ++k) {
-            Group &g = gr[k];
-            hipStream_t gs = ctx->group_stream[k];
-            group_tasks(g, js, shapes, s1_of, rest, g.tail);
-            // same task count as the chunk's table, so no reallocation while
-            // the group's earlier launches still read it
-            GD_HIP(upload(g.tasks_dev, g.tail.data(), sizeof(DevTask) * g.tail.size(), gs));
-            GD_HIP(launch_mc(t.dt, t.sc, jobs_dev.as<DevJob>(), g.tasks_dev.as<DevTask>(), (int)g.tail.size(), W,
-                             max_rep, comp, records, gs, g.step_base()));
-        }
-    if (int rc = join_groups(ctx, s, gr.size(), ej)) return rc;
+        if (int rc = enqueue_mc(ctx, t, jobs_dev.as<DevJob>(), tail_dev.as<DevTask>(), (int)tail.size(), rest, W,
+                                max_rep, comp, records, s, step_base))
+            return rc;
     GD_HIP(hipEventRecord(e1.e, s));
     GD_HIP(hipEventSynchronize(e1.e));
     float ms = 0.f;

@@ -54,6 +54,7 @@ struct TargetRes {
 // times `slices` protein-point slices.
 struct Shape {
     int nrep, slot_waves, slices, waves;
+    int layout = 0; // TaskLayout bits
 };
 
 struct JobRes {
@@ -92,6 +93,9 @@ struct gd_ctx {
     int next_id = 1;
     // launch scratch reused across calls
     gd::DevBuf task_buf, job_buf, pose_rep, pose_move, pose_out, counter;
+    // the batched form's work queue (counter + done per task), exchange
+    // ladder list and graph step/round counters
+    gd::DevBuf queue, ladders, bases;
     // streams of the batched form's workgroup groups (created on first use,
     // default priority)
     static constexpr int kGroups = 2;
@@ -107,6 +111,11 @@ int check_stream_range(const JobRes &j, long s_end);
 int check_complex_arrays(const gd_Complex *cx, bool target, bool job);
 int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std::vector<DevTask> &tasks,
               int waves, int max_rep, bool compensated, bool records);
+// workgroups of shape (waves, max_rep) resident on the whole GPU at once
+int resident_slots(gd_ctx *ctx, int waves, int max_rep);
+// step slices per task for a persistent launch of ntask tasks on `slots`
+// resident workgroups, each task running `steps` steps
+int choose_slices(int ntask, int slots, int steps);
 // creates the context's group streams on first use
 int ensure_group_streams(gd_ctx *ctx);
 } // namespace gd

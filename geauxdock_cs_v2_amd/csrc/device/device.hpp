@@ -139,9 +139,28 @@ struct DevJob {
 struct DevTask {
     int job, rep0, nrep, prt;
     int s1, s2max, slices, slot_waves;
+    int layout; // TaskLayout bits
 };
 
+// Where the KDE and MCS sums of a task run (autotuner candidates; the
+// results differ only by summation order).  Spread: slice s of a replica's
+// workers takes KDE points kr.x + s, kr.x + s + S, ... and MCS rows
+// a S + s, a S + s + S N_L, ... (every slice shares the gathers); slice 0:
+// the first point slice alone takes the atom's whole KDE run and rows
+// a, a + N_L, ... (the other slices skip them).
+enum TaskLayout { LAYOUT_KDE_SLICE0 = 1, LAYOUT_MCS_SLICE0 = 2 };
+
 enum KernelMode { MODE_MC = 0, MODE_ENERGY = 1 };
+
+// Work queue of a persistent launch (see mc_kernel): items u = k * ntask + t
+// run step slice k of task t.  counter (next item) and done[ntask] (slices
+// finished per task) start at zero; a null counter launches one workgroup
+// per task instead.
+struct DevQueue {
+    int *counter;
+    int *done;
+    int ntask, nslice;
+};
 
 // LDS bytes a workgroup of `waves` waves needs for `nrep` replicas
 size_t mc_lds_bytes(int waves, int nrep);
@@ -154,6 +173,19 @@ hipError_t launch_mc(const DevTarget &tg, const DevScalars &sc, const DevJob *jo
                      int ntask, int waves, int max_rep, bool compensated, bool keep_records,
                      hipStream_t stream, const int *step_base);
 
+// Persistent form: `grid` workgroups walk the queue q (q.ntask tasks in
+// q.nslice step slices each).  The queue's counter and done[] must be zero
+// at launch (launch_queue_reset clears them on the stream, with a kernel, so
+// the sequence can be captured in a graph).
+hipError_t launch_mc_queue(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
+                           const DevQueue &q, int grid, int waves, int max_rep, bool compensated,
+                           bool keep_records, hipStream_t stream, const int *step_base);
+hipError_t launch_queue_reset(const DevQueue &q, hipStream_t stream);
+
+// Workgroups of the Monte Carlo kernel resident on one CU for this shape
+// (occupancy of the compensated batched instantiation; the others match).
+int mc_blocks_per_cu(int waves, int max_rep);
+
 // Energy mode: poses i = task.rep0 (nrep == 1), replica pose_rep[i], move pose_move[6i..]
 hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
                          int ntask, int waves, bool compensated, const int *pose_rep, const float *pose_move,
@@ -164,5 +196,11 @@ hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob
 // round_base (device int, may be null) is added to `round`.
 hipError_t launch_exchange(const DevJob *job_dev, int ladder0, int n_ladder, int n_lig, int n_tmp, int round,
                            int *accept_count, hipStream_t stream, const int *round_base);
+
+// One exchange round over the ladders of several jobs in one launch: entry i
+// of `ladders` is {job, ladder, round offset, 0}: ladder `ladder` of jobs[job]
+// at round `round offset` + *round_base.
+hipError_t launch_exchange_list(const DevJob *jobs, const int4 *ladders, int n, hipStream_t stream,
+                                const int *round_base);
 
 } // namespace gd

@@ -21,6 +21,8 @@
 // sum and per-atom hydrophobic restraint; then a per-replica sum,
 // normalisation, weighted total, Metropolis accept and record append
 // (kernel_cpp_montecarlo_soa.cpp:333-390) by one thread per replica.
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace gd {
@@ -336,19 +338,55 @@ size_t mc_lds_bytes(int waves, int nrep)
 }
 
 // At least 6 waves per SIMD (at most 80 VGPRs): two 12-wave workgroups share
-// a CU.  The spills this costs are outside the pair loops.  RECORDS: append
-// every accepted step to the job's record window (the plain chunk keeps
-// them; batched runs usually do not, and the instantiations have distinct
-// names in profiles).
+// a CU.  RECORDS: append every accepted step to the job's record window (the
+// plain chunk keeps them; batched runs usually do not, and the
+// instantiations have distinct names in profiles).
+//
+// Work distribution.  Without a queue (q.counter null) workgroup b runs task
+// b for all of its steps.  With a queue the grid is the number of
+// workgroups resident at once and each workgroup loops: it takes the next
+// item u from the atomic counter, runs step slice k = u / ntask of task
+// t = u % ntask (tasks in decreasing cost order, so heavy ones start first;
+// slice k covers steps [k s2max / nslice, (k+1) s2max / nslice) of the task)
+// and publishes the slice with a release store to done[t].  Slice k of a
+// task waits (acquire) for done[t] == k, i.e. for the previous slice's
+// replica states.  Items are taken in increasing order and a waiting
+// workgroup only waits on an item taken earlier by a running workgroup,
+// which itself waits only on earlier items, so the waits end.  Step slices
+// let the last round of a launch spread over every resident workgroup
+// instead of leaving CUs idle behind a partly filled round of whole tasks.
 template <bool KAHAN, int MODE, bool RECORDS>
 __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc, const DevJob *__restrict__ jobs,
                                                   const DevTask *__restrict__ tasks,
                                                   const int *__restrict__ step_base,
                                                   const int *__restrict__ pose_rep,
-                                                  const float *__restrict__ pose_move, float *__restrict__ pose_out)
+                                                  const float *__restrict__ pose_move, float *__restrict__ pose_out,
+                                                  DevQueue q)
 {
     extern __shared__ float lds[];
-    const DevTask task = tasks[blockIdx.x];
+    __shared__ int s_item;
+    const int n_items = q.counter ? q.ntask * q.nslice : 0;
+    for (int pass = 0;; ++pass) {
+    int ti = blockIdx.x, kslice = 0, nslice = 1;
+    if (q.counter) {
+        if (threadIdx.x == 0) s_item = atomicAdd(q.counter, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_item);
+        if (u >= n_items) break;
+        ti = u % q.ntask;
+        kslice = u / q.ntask;
+        nslice = q.nslice;
+        if (kslice > 0) {
+            // the previous slice of this task: its replica states and records
+            if (threadIdx.x == 0)
+                while (__hip_atomic_load(q.done + ti, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < kslice)
+                    __builtin_amdgcn_s_sleep(4);
+            __syncthreads();
+        }
+    } else if (pass > 0) {
+        break;
+    }
+    const DevTask task = tasks[ti];
     const DevJob &J = jobs[task.job];
     const int W = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -373,15 +411,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     float *prop = st + (size_t)R * 16;     // [R][8] proposed move, accept uniform
     int *ist = (int *)(prop + (size_t)R * 8); // [R][kIst] next_entry, is_accept, step, idx_tmp, accepted moves
 
+    // this item's steps: slice kslice of the task's s2max steps
+    const int sbeg = (int)(((long long)task.s2max * kslice) / nslice);
+    const int send = (int)(((long long)task.s2max * (kslice + 1)) / nslice);
     const int s1 = task.s1 + (step_base ? *step_base : 0);
-    const int s2max = task.s2max;
 
     if (MODE == MODE_MC) {
         for (int i = threadIdx.x; i < R; i += blockDim.x) {
             const gd_ReplicaMC &rp = J.rep[task.rep0 + i];
             for (int k = 0; k < 6; ++k) st[i * 16 + k] = rp.movematrix[k];
             for (int k = 0; k < GD_MAXWEI; ++k) st[i * 16 + 6 + k] = rp.energy[k];
-            ist[i * kIst + 0] = 0;
+            ist[i * kIst + 0] = RECORDS && kslice > 0 ? J.next_entry[task.rep0 + i] : 0;
             ist[i * kIst + 1] = rp.is_accept;
             ist[i * kIst + 2] = rp.step;
             ist[i * kIst + 3] = rp.idx_tmp;
@@ -415,7 +455,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     int seg_e = seg_b;
     while (seg_e < tg.nseg && tg.seg[seg_e].y < hi) ++seg_e;
 
-    for (int s2 = 0; s2 < s2max; ++s2) {
+    for (int s2 = sbeg; s2 < send; ++s2) {
         const int step = s1 + s2;
         float m[6];
         float uacc = 0.0f;
@@ -624,10 +664,12 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // so the rows of a small table still fall on every slice
         float ks = 0.f, mcs = 0.f;
         const int2 kr = J.krange[a];
-        if (active) {
-            int k = kr.x + slice;
-            for (; k + S < kr.y; k += 2 * S) {
-                const float4 K0 = J.kde[k], K1 = J.kde[k + S];
+        const bool kde0 = task.layout & LAYOUT_KDE_SLICE0, mcs0 = task.layout & LAYOUT_MCS_SLICE0;
+        if (active && (!kde0 || slice == 0)) {
+            const int KS = kde0 ? 1 : S;
+            int k = kr.x + (kde0 ? 0 : slice);
+            for (; k + KS < kr.y; k += 2 * KS) {
+                const float4 K0 = J.kde[k], K1 = J.kde[k + KS];
                 const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
                 const float bx = me.x - K1.x, by = me.y - K1.y, bz = me.z - K1.z;
                 ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
@@ -638,8 +680,11 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
                 ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
             }
+        }
+        if (active && (!mcs0 || slice == 0)) {
+            const int MS = mcs0 ? 1 : S;
             const float4 *lig2 = J.lig2 + (size_t)idx_lig * NL;
-            for (int row = a * S + slice; row < J.nrow; row += S * NL) {
+            for (int row = a * MS + (mcs0 ? 0 : slice); row < J.nrow; row += MS * NL) {
                 const int rb = J.mrow[row], re = J.mrow[row + 1];
                 float s = 0.f;
 #pragma unroll 4
@@ -796,18 +841,20 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             J.mc_acc[task.rep0 + i] += ist[i * kIst + 4]; // this workgroup owns the replica
         }
     }
+    // every wave's stores of the replica states are done before the slice
+    // is published and before the LDS state is reused by the next item
+    __syncthreads();
+    if (q.counter && threadIdx.x == 0)
+        __hip_atomic_store(q.done + ti, kslice + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } // item loop
 }
 
-// One thread per ladder (ligand conformation l, protein conformation p):
-// temperature replica exchange, see gd_exchange in geauxdock_amd.h.
-__global__ void exchange_kernel(const DevJob *__restrict__ job, int ladder0, int n_ladder, int n_lig, int n_tmp,
-                                int round, const int *__restrict__ round_base, int *__restrict__ accept_count)
+// Temperature replica exchange of ladder L (ligand conformation l, protein
+// conformation p, L = p n_lig + l) at round rnd, see gd_exchange in
+// geauxdock_amd.h.  Returns the accepted swaps.
+__device__ int exchange_ladder(const DevJob &J, int L, int rnd)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_ladder) return;
-    const int L = ladder0 + i;
-    const int rnd = round + (round_base ? *round_base : 0);
-    const DevJob &J = *job;
+    const int n_lig = J.n_lig, n_tmp = J.n_tmp;
     const int p = L / n_lig, l = L - p * n_lig;
     int holder[64];
     for (int t = 0; t < n_tmp; ++t) {
@@ -833,29 +880,100 @@ __global__ void exchange_kernel(const DevJob *__restrict__ job, int ladder0, int
             ++accepted;
         }
     }
+    return accepted;
+}
+
+// One thread per ladder of one job, over a ladder range.
+__global__ void exchange_kernel(const DevJob *__restrict__ job, int ladder0, int n_ladder, int round,
+                                const int *__restrict__ round_base, int *__restrict__ accept_count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_ladder) return;
+    const int rnd = round + (round_base ? *round_base : 0);
+    const int accepted = exchange_ladder(*job, ladder0 + i, rnd);
     if (accept_count && accepted) atomicAdd(accept_count, accepted);
 }
+
+// One thread per entry {job, ladder, round offset} of a ladder list spanning
+// several jobs.
+__global__ void exchange_list_kernel(const DevJob *__restrict__ jobs, const int4 *__restrict__ ladders, int n,
+                                     const int *__restrict__ round_base)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 e = ladders[i];
+    (void)exchange_ladder(jobs[e.x], e.y, e.z + (round_base ? *round_base : 0));
+}
+
+namespace {
+
+template <bool KAHAN, bool RECORDS>
+hipError_t launch_mc_impl(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
+                          const DevQueue &q, int grid, int waves, int max_rep, hipStream_t stream,
+                          const int *step_base)
+{
+    const size_t lds = mc_lds_bytes(waves, max_rep);
+    hipLaunchKernelGGL((mc_kernel<KAHAN, MODE_MC, RECORDS>), dim3(grid), dim3(waves * 64), lds, stream, tg, sc, jobs,
+                       tasks, step_base, nullptr, nullptr, nullptr, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_mc_any(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
+                         const DevQueue &q, int grid, int waves, int max_rep, bool compensated, bool keep_records,
+                         hipStream_t stream, const int *step_base)
+{
+    if (grid <= 0) return hipSuccess;
+    if (compensated && keep_records)
+        return launch_mc_impl<true, true>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+    if (compensated)
+        return launch_mc_impl<true, false>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+    if (keep_records)
+        return launch_mc_impl<false, true>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+    return launch_mc_impl<false, false>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+}
+
+__global__ void queue_reset_kernel(int *counter, int *done, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *counter = 0;
+    if (i < n) done[i] = 0;
+}
+
+} // namespace
 
 hipError_t launch_mc(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks, int ntask,
                      int waves, int max_rep, bool compensated, bool keep_records, hipStream_t stream,
                      const int *step_base)
 {
-    if (ntask <= 0) return hipSuccess;
-    const size_t lds = mc_lds_bytes(waves, max_rep);
-    const dim3 grid(ntask), block(waves * 64);
-    if (compensated && keep_records)
-        hipLaunchKernelGGL((mc_kernel<true, MODE_MC, true>), grid, block, lds, stream, tg, sc, jobs, tasks, step_base,
-                           nullptr, nullptr, nullptr);
-    else if (compensated)
-        hipLaunchKernelGGL((mc_kernel<true, MODE_MC, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
-                           step_base, nullptr, nullptr, nullptr);
-    else if (keep_records)
-        hipLaunchKernelGGL((mc_kernel<false, MODE_MC, true>), grid, block, lds, stream, tg, sc, jobs, tasks,
-                           step_base, nullptr, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((mc_kernel<false, MODE_MC, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
-                           step_base, nullptr, nullptr, nullptr);
+    const DevQueue none{nullptr, nullptr, ntask, 1};
+    return launch_mc_any(tg, sc, jobs, tasks, none, ntask, waves, max_rep, compensated, keep_records, stream,
+                         step_base);
+}
+
+hipError_t launch_mc_queue(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
+                           const DevQueue &q, int grid, int waves, int max_rep, bool compensated,
+                           bool keep_records, hipStream_t stream, const int *step_base)
+{
+    if (!q.counter || !q.done || q.ntask <= 0 || q.nslice <= 0) return hipErrorInvalidValue;
+    return launch_mc_any(tg, sc, jobs, tasks, q, std::min(grid, q.ntask * q.nslice), waves, max_rep, compensated,
+                         keep_records, stream, step_base);
+}
+
+hipError_t launch_queue_reset(const DevQueue &q, hipStream_t stream)
+{
+    const int n = std::max(q.ntask, 1);
+    hipLaunchKernelGGL(queue_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, q.counter, q.done, q.ntask);
     return hipGetLastError();
+}
+
+int mc_blocks_per_cu(int waves, int max_rep)
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mc_kernel<true, MODE_MC, false>, waves * 64,
+                                                     mc_lds_bytes(waves, max_rep)) != hipSuccess ||
+        blocks <= 0)
+        blocks = std::max(1, 24 / waves); // 6 waves per SIMD, four SIMDs
+    return blocks;
 }
 
 hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
@@ -867,20 +985,31 @@ hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob
     const dim3 grid(ntask), block(waves * 64);
     if (compensated)
         hipLaunchKernelGGL((mc_kernel<true, MODE_ENERGY, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
-                           nullptr, pose_rep, pose_move, pose_out);
+                           nullptr, pose_rep, pose_move, pose_out, DevQueue{nullptr, nullptr, ntask, 1});
     else
         hipLaunchKernelGGL((mc_kernel<false, MODE_ENERGY, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
-                           nullptr, pose_rep, pose_move, pose_out);
+                           nullptr, pose_rep, pose_move, pose_out, DevQueue{nullptr, nullptr, ntask, 1});
     return hipGetLastError();
 }
 
 hipError_t launch_exchange(const DevJob *job_dev, int ladder0, int n_ladder, int n_lig, int n_tmp, int round,
                            int *accept_count, hipStream_t stream, const int *round_base)
 {
+    (void)n_lig;
     if (n_ladder <= 0 || n_tmp < 2) return hipSuccess;
     const int bs = 64;
     hipLaunchKernelGGL(exchange_kernel, dim3((n_ladder + bs - 1) / bs), dim3(bs), 0, stream, job_dev, ladder0, n_ladder,
-                       n_lig, n_tmp, round, round_base, accept_count);
+                       round, round_base, accept_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_exchange_list(const DevJob *jobs, const int4 *ladders, int n, hipStream_t stream,
+                                const int *round_base)
+{
+    if (n <= 0) return hipSuccess;
+    const int bs = 64;
+    hipLaunchKernelGGL(exchange_list_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, stream, jobs, ladders, n,
+                       round_base);
     return hipGetLastError();
 }
 

@@ -31,22 +31,52 @@ def flops_per_pose(cx):
     return 60 * nl * npt + 12 * kde_pairs + 9 * int(ncol.sum()) + 4 * s["mcs_nrow"] + 36 * nl + 60
 
 
-def cpu_model():
+def _lscpu():
     try:
         out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
-        for line in out.splitlines():
-            if line.startswith("Model name:"):
-                return line.split(":", 1)[1].strip()
     except OSError:
-        pass
-    return platform.processor() or "unknown"
+        return {}
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    return kv
+
+
+def cpu_model():
+    return _lscpu().get("Model name") or platform.processor() or "unknown"
+
+
+def affinity_cpus():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+# CPU threads one GPU's share of the measurement hosts allows (those hosts
+# give each GPU 16 host cores, whatever nproc reports)
+HOST_THREAD_CAP = 16
 
 
 def host_threads():
-    """Threads this process may use: the affinity mask, capped at 16 (the
-    CPU share of one GPU on the measurement hosts)."""
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    """Threads the CPU baseline uses: the affinity mask, capped at
+    HOST_THREAD_CAP."""
+    return max(1, min(HOST_THREAD_CAP, affinity_cpus()))
+
+
+def host_info():
+    """lscpu model, sockets, cores per socket and threads per core, the
+    affinity mask's CPU count, the thread cap and the OpenMP binding."""
+    kv = _lscpu()
+
+    def num(key):
+        try:
+            return int(kv.get(key, ""))
+        except ValueError:
+            return None
+    return {"model": kv.get("Model name") or platform.processor() or "unknown", "sockets": num("Socket(s)"),
+            "cores_per_socket": num("Core(s) per socket"), "threads_per_core": num("Thread(s) per core"),
+            "logical_cpus": num("CPU(s)"), "affinity_cpus": affinity_cpus(), "thread_cap": HOST_THREAD_CAP,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}

@@ -4,8 +4,9 @@ Imported by tests/, by __graft_entry__.smoke() (as the checker) and by the
 CPU-baseline leg of bench.py.  The product never imports this module.
 
 liboracle.so  : the repository's C restatement (gd_oracle.c)
-_ref/libgdref.so : the reference's own MonteCarlo_d and table builders,
-                   compiled from its sources (present only where it was built)
+_ref/libgdref.so : the reference's own MonteCarlo_d, table builders and
+                   temperature ladder, compiled from its sources (present only
+                   where it was built)
 """
 import ctypes as C
 import os
@@ -68,6 +69,9 @@ def ref():
         lib.gdref_montecarlo.argtypes = [C.c_void_p, C.POINTER(D.ReplicaMC), C.POINTER(D.ReplicaMC),
                                          C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint64, C.c_int]
         lib.gdref_montecarlo.restype = C.c_int
+        lib.gdref_set_temperature.argtypes = [C.c_int, C.c_float, C.c_float, C.POINTER(C.c_float),
+                                              C.POINTER(C.c_int)]
+        lib.gdref_set_temperature.restype = C.c_int
         _ref = lib
     return _ref
 
@@ -115,6 +119,16 @@ def montecarlo(view, reps, s1, s2max, seed, cap=None, r0=0, r1=None, nthreads=0)
 
 def exchange(view, reps, rnd, seed):
     return oracle().gdo_exchange(C.byref(view), reps, rnd, seed)
+
+
+def ref_ladder(n, t_low, t_high):
+    """The reference's own Loader::set_temperature: (minus_beta, order) of an
+    n-rung ladder between t_low and t_high (float32 arrays)."""
+    mb = np.zeros(n, np.float32)
+    order = np.zeros(n, np.int32)
+    if ref().gdref_set_temperature(n, t_low, t_high, _f32p(mb), _i32p(order)) != 0:
+        raise ValueError("ladder size out of the reference's range")
+    return mb, order
 
 
 class RefComplex:

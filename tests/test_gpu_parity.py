@@ -424,9 +424,11 @@ def test_mclog_counters(lib, ctx, kat_complex11):
 
 
 def test_bad_replica_layouts_are_refused(lib, ctx, kat_complex11):
-    """Replica tables whose ladders do not hold each temperature once, or whose
-    replicas sit out of their (ligand, protein conformation) place, are refused
-    at upload and at write, before any launch reads them."""
+    """Replica tables whose ladders do not hold each temperature once, whose
+    replicas sit out of their (ligand, protein conformation) place, or whose
+    idx_rep values (the stream keys and explicit-word rows) are not a
+    permutation of the replica indices, are refused at upload and at write,
+    before any launch reads them."""
     cx = kat_complex11[1]
     n = cx.size["n_rep"]
     n_lig, n_tmp = cx.size["n_lig"], cx.size["n_tmp"]
@@ -439,12 +441,19 @@ def test_bad_replica_layouts_are_refused(lib, ctx, kat_complex11):
             reps[n_lig].idx_tmp = reps[0].idx_tmp  # ladder (0, 0): temperatures 0, 0, 2, 3
         elif kind == "protein conformation out of place":
             reps[0].idx_prt = 1
+        elif kind == "replica index out of range":
+            reps[n - 1].idx_rep = n  # would index explicit stream words beyond the buffer
+        elif kind == "negative replica index":
+            reps[1].idx_rep = -1
+        elif kind == "duplicate replica index":
+            reps[1].idx_rep = reps[0].idx_rep  # two replicas would share one stream
         else:  # ligand conformation out of place
             reps[0].idx_lig = 1 % n_lig if n_lig > 1 else 0
             reps[0].idx_tmp = n_tmp  # out of range when there is one conformation
         return reps
 
-    for kind in ("duplicate temperature", "protein conformation out of place", "ligand conformation out of place"):
+    for kind in ("duplicate temperature", "protein conformation out of place", "ligand conformation out of place",
+                 "replica index out of range", "negative replica index", "duplicate replica index"):
         say(f"case: upload and write refuse a replica table with a {kind}")
         reps = broken(kind)
         view = D.Complex.from_buffer_copy(cx.view)

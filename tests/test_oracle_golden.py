@@ -81,18 +81,27 @@ def test_loader_sizes_test5(kat_complex11):
 
 
 def _a2_rows():
+    """Every row of the reference's size table; each row's ligand files are
+    fixtures, so a missing file fails its row instead of dropping it."""
     rows = []
     with open(os.path.join(GOLDEN, "a2_sizes.csv")) as f:
         for line in f:
             t = line.strip().split(",")
-            if os.path.exists(os.path.join(ASTEX, t[0] + ".sdf")):
+            if t and t[0]:
                 rows.append((t[0], [int(x) for x in t[1:]]))
     return rows
+
+
+def test_a2_table_is_complete():
+    """all 18 ligand rows of reference src/main_mpi/a2.csv:2-19"""
+    assert len(_a2_rows()) == 18
 
 
 @pytest.mark.parametrize("lig,want", _a2_rows())
 def test_loader_sizes_a2(lib, lig, want):
     """reference src/main_mpi/a2.csv:2-19 against 1a07C.pdb, one temperature."""
+    for ext in (".sdf", "-0.8.ff"):
+        assert os.path.exists(os.path.join(ASTEX, lig + ext)), f"fixture {lig}{ext} missing"
     raw = L.load_job(os.path.join(ASTEX, "1a07C.pdb"), os.path.join(ASTEX, lig + ".sdf"),
                      os.path.join(ASTEX, lig + "-0.8.ff"), lig)
     s = raw.stage().size
