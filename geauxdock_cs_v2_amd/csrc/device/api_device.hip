@@ -391,6 +391,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         t_prot_r = prot_r;
     }
     std::vector<float2> pa((size_t)kCenters * np);
+    std::vector<float> pw((size_t)kCenters * np);
     std::vector<float4> pt((size_t)kCenters * nconf * np);
     std::vector<int> ladder((size_t)kCenters * nseg * kFarRungs), ladder_in((size_t)kCenters * nseg * kFarRungs);
     std::vector<double> dmin(np), dmax(np);
@@ -422,6 +423,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             v.x = 0.5f * hpcw[s]; // the factored contact polynomial leaves a factor 1/2 to the weight
             std::memcpy(&v.y, &cx->prt_t[s], 4);
             pa[(size_t)kc * np + i] = v;
+            pw[(size_t)kc * np + i] = v.x;
         }
         for (int c = 0; c < nconf; ++c)
             for (int i = 0; i < np; ++i) {
@@ -458,6 +460,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     } drop{ctx, id};
     GD_HIP(upload(t.pt, pt.data(), sizeof(float4) * pt.size(), ctx->stream));
     GD_HIP(upload(t.pa, pa.data(), sizeof(float2) * pa.size(), ctx->stream));
+    GD_HIP(upload(t.pw, pw.data(), sizeof(float) * pw.size(), ctx->stream));
     GD_HIP(upload(t.pp, pp.data(), sizeof(float) * pp.size(), ctx->stream));
     GD_HIP(upload(t.seg, seg.data(), sizeof(int4) * seg.size(), ctx->stream));
     GD_HIP(upload(t.ladder, ladder.data(), sizeof(int) * ladder.size(), ctx->stream));
@@ -473,6 +476,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.prot_r = (float)t_prot_r + 1e-3f; // rounded up: a rung at or beyond it holds every point
     t.dt.pt = t.pt.as<float4>();
     t.dt.pa = t.pa.as<float2>();
+    t.dt.pw = t.pw.as<float>();
     t.dt.pp = t.pp.as<float>();
     t.dt.seg = t.seg.as<int4>();
     t.dt.npoint = np;
@@ -512,7 +516,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         t.seg_host.push_back(g.y);
         t.seg_host.push_back(g.z);
     }
-    t.bytes = t.pt.bytes + t.pa.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes + t.ladder_in.bytes;
+    t.bytes = t.pt.bytes + t.pa.bytes + t.pw.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes + t.ladder_in.bytes;
     t.dev_index.assign(np, 0);
     for (int i = 0; i < np; ++i) t.dev_index[ord[i]] = i;
     t.prt_t.assign(cx->prt_t, cx->prt_t + np);

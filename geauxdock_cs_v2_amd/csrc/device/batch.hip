@@ -208,6 +208,9 @@ int enqueue_mc(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const De
     if (int rc = ensure_queue(ctx, ntask, q)) return rc;
     const int slots = resident_slots(ctx, waves, max_rep);
     q.nslice = choose_slices(ntask, slots, s2max);
+    ctx->last_plan[0] = ntask;
+    ctx->last_plan[1] = std::min(slots, ntask * q.nslice);
+    ctx->last_plan[2] = q.nslice;
     GD_HIP(launch_queue_reset(q, s));
     GD_HIP(launch_mc_queue(t.dt, t.sc, jobs_dev, tasks_dev, q, slots, waves, max_rep, comp, records, s, step_base));
     return GD_OK;
@@ -505,6 +508,13 @@ extern "C" int gd_job_shape(gd_ctx *ctx, int job_id, int *shape)
     shape[1] = s.slot_waves;
     shape[2] = s.slices;
     shape[3] = s.nrep;
+    return GD_OK;
+}
+
+extern "C" int gd_batch_plan(gd_ctx *ctx, int *plan)
+{
+    if (!ctx || !plan) return fail(GD_ERR_ARG, "null argument");
+    for (int k = 0; k < 3; ++k) plan[k] = ctx->last_plan[k];
     return GD_OK;
 }
 

@@ -39,7 +39,7 @@ struct DevBuf {
 };
 
 struct TargetRes {
-    DevBuf pt, pa, pp, seg, ladder, ladder_in;
+    DevBuf pt, pa, pw, pp, seg, ladder, ladder_in;
     DevTarget dt{};
     DevScalars sc{};
     int n_prt = 0, npoint = 0;
@@ -96,6 +96,8 @@ struct gd_ctx {
     // the batched form's work queue (counter + done per task), exchange
     // ladder list and graph step/round counters
     gd::DevBuf queue, ladders, bases;
+    // work queue of the last persistent launch: tasks, grid, step slices
+    int last_plan[3] = {0, 0, 0};
     // streams of the batched form's workgroup groups (created on first use,
     // default priority)
     static constexpr int kGroups = 2;

@@ -10,6 +10,7 @@
 //                                     parameters and terms; copy k orders each
 //                                     segment by distance from ladder centre k
 //           pa   float2[kCenters][npad] {hpc weight = (cdc ? hpp : 0) / 2, point type}
+//           pw   float[kCenters][npad] the hpc weights of pa alone
 //           seg  int4[nseg]           {point type, begin, end, hydrophobic flag}
 //           ladder int[kCenters][nseg][kFarRungs] first point of each segment at
 //                                     or beyond each distance rung from the
@@ -86,6 +87,7 @@ struct DevTarget {
     // pocket centre); the segments themselves are the same in every copy
     const float4 *pt;  // [kCenters][nconf][npad] {x, y, z, ele}, pocket-centred frame
     const float2 *pa;  // [kCenters][npad] {hydrophobic weight / 2, point type (int bits)}
+    const float *pw;   // [kCenters][npad] hydrophobic weight / 2 alone, contiguous for wide scalar loads
     const float *pp;   // [31][24][PP_N] type-pair parameters
     const int4 *seg;   // [nseg] {point type, begin, end, hydrophobic flag}
     const int *ladder; // [kCenters][nseg][kFarRungs] first point of the segment at distance >= rung from the centre

@@ -129,15 +129,17 @@ __device__ __forceinline__ float ele_core_delta(float d2, float rinv, float ele,
 }
 
 // Soft-core Lennard-Jones (:62-71), times c (c = lj0, or 1 where lj0 = 0):
-// with X = c pp1, c (pp1 - pp2) / pp4 = (X - c p2a r^6) / (1 + X (lj0 / c)(1 + lj1 d2)).
-__device__ __forceinline__ void lj_term(float d2, float rinv, const PairParams &q, const DevScalars &sc, Sums &s)
+// with X = c pp1 = c p1a / d^9, c (pp1 - pp2) / pp4 = (X - c p2a / d^6) /
+// (1 + X L), L = (lj0 / c)(1 + lj1 d2); numerator and denominator times d^9:
+// (c p1a - c p2a d^3) / (d^9 + c p1a L), seven operations and one reciprocal
+// from d (the powers of 1/d take three more).  c p1a > 0 for every type pair
+// of the parameter table and L > 0, so the denominator is positive.
+__device__ __forceinline__ void lj_term(float d2, float d, const PairParams &q, const DevScalars &sc, Sums &s)
 {
-    const float r2 = rinv * rinv;
-    const float r3 = r2 * rinv;
-    const float r6 = r3 * r3;
-    const float X = q.p1a * (r6 * r3);
-    const float pp4 = fmaf(X, fmaf(sc.lj_b, d2, sc.lj_a), 1.0f);
-    s.v = fmaf(fmaf(-q.p2a, r6, X), __builtin_amdgcn_rcpf(pp4), s.v);
+    const float d3 = d2 * d;
+    const float d9 = (d3 * d3) * d3;
+    const float den = fmaf(q.p1a, fmaf(sc.lj_b, d2, sc.lj_a), d9);
+    s.v = fmaf(fmaf(-q.p2a, d3, q.p1a), __builtin_amdgcn_rcpf(den), s.v);
 }
 
 // Terms of one pair once d2 and 1/d are known, electrostatics aside: the
@@ -155,7 +157,7 @@ __device__ __forceinline__ void pair_terms(float d2, float rinv, float hw, const
         const float u = __builtin_amdgcn_fmed3f(fmaf(-d2, 1.0f / 81.0f, 1.0f), 0.0f, 1.0f);
         s.q = fmaf(hw, (u * u) * fmaf(u, u, u), s.q);
     }
-    lj_term(d2, rinv, q, sc, s);
+    lj_term(d2, d, q, sc, s);
     // contact potential pmf1 / (1 + exp((6 - d/2)(d - pmf0))) (:95-98)
     const float xe = fmaf(sc.pmf_a, d2, fmaf(q.pmfb, d, q.pmfc));
     s.p = fmaf(q.pmf1, __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(xe)), s.p);
@@ -177,18 +179,23 @@ __device__ __forceinline__ void pair_terms(float d2, float rinv, float hw, const
 // (closer than 1 / el1: clashes, rare), whose corrections are skipped
 // otherwise.
 template <bool HPC, bool HDB>
-__device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_, int pb, int pe, const Slot &me,
+__device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_, int pb, int pe, const Slot &me,
                                            const PairParams &q, const DevScalars &sc, Sums &s)
 {
     const cfloat *ptc = (const cfloat *)ptc_;
-    const cfloat *pa = (const cfloat *)pa_; // {weight, type} pairs
+    const cfloat *pw = (const cfloat *)pw_; // hydrophobic weights
     int p = pb;
     for (; p + 4 <= pe; p += 4) {
+        // one base address per group: the four points are one 64-byte
+        // scalar load, their weights one 16-byte load
+        const cfloat *g4 = ptc + 4 * p;
+        const cfloat *w4 = pw + p;
         float4 P[4];
-        float d2[4], rinv[4];
+        float d2[4], rinv[4], w[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            P[j] = ld_point(ptc, p + j);
+            P[j] = make_float4(g4[4 * j], g4[4 * j + 1], g4[4 * j + 2], g4[4 * j + 3]);
+            w[j] = HPC ? w4[j] : 0.0f;
             const float dx = me.x - P[j].x, dy = me.y - P[j].y, dz = me.z - P[j].z;
             d2[j] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             rinv[j] = __builtin_amdgcn_rsqf(d2[j]);
@@ -196,7 +203,7 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_
         float ge = 0.0f; // the group's electrostatics
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            pair_terms<HPC, HDB>(d2[j], rinv[j], HPC ? pa[2 * (p + j)] : 0.0f, q, sc, s);
+            pair_terms<HPC, HDB>(d2[j], rinv[j], w[j], q, sc, s);
             ge = fmaf(P[j].w, rinv[j], ge);
         }
         // core corrections where some lane of the wave has a core pair in the group
@@ -212,7 +219,7 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float2 *pa_
         const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float rinv = __builtin_amdgcn_rsqf(d2);
-        pair_terms<HPC, HDB>(d2, rinv, HPC ? pa[2 * p] : 0.0f, q, sc, s);
+        pair_terms<HPC, HDB>(d2, rinv, HPC ? pw[p] : 0.0f, q, sc, s);
         kahan_add(s.e, s.ec, fmaf(P4.w, rinv, ele_core_delta(d2, rinv, P4.w, sc)));
     }
 }
@@ -233,7 +240,7 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int p
             const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             const float rinv = __builtin_amdgcn_rsqf(d2);
-            lj_term(d2, rinv, q, sc, s);
+            lj_term(d2, d2 * rinv, q, sc, s);
             ge = fmaf(P4.w, rinv, ge);
         }
         kahan_add(s.e, s.ec, ge);
@@ -243,7 +250,7 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int p
         const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float rinv = __builtin_amdgcn_rsqf(d2);
-        lj_term(d2, rinv, q, sc, s);
+        lj_term(d2, d2 * rinv, q, sc, s);
         kahan_add(s.e, s.ec, P4.w * rinv);
     }
     s.p = fmaf(q.pmf1, (float)(pe - pb), s.p);
@@ -501,6 +508,52 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         me.z = r20 * L3.x + r21 * L3.y + r22 * L3.z + tz;
         me.tl = tl;
 
+        // KDE and MCS first: the rotation is then dead before the pair loops,
+        // which need the registers.
+        // KDE over the atom type's run of the type-sorted table
+        // (energy_kde_v1.cpp:1-77): slice s takes points kr.x + s, kr.x + s + S, ...
+        // MCS rows over the file-order atom set (energy_mcs_v3_ell.cpp):
+        // worker a * S + s of the replica takes rows a * S + s, + S * N_L, ...,
+        // so the rows of a small table still fall on every slice
+        float ks = 0.f, mcs = 0.f;
+        const int2 kr = J.krange[a];
+        const bool kde0 = task.layout & LAYOUT_KDE_SLICE0, mcs0 = task.layout & LAYOUT_MCS_SLICE0;
+        if (active && (!kde0 || slice == 0)) {
+            const int KS = kde0 ? 1 : S;
+            int k = kr.x + (kde0 ? 0 : slice);
+            for (; k + KS < kr.y; k += 2 * KS) {
+                const float4 K0 = J.kde[k], K1 = J.kde[k + KS];
+                const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
+                const float bx = me.x - K1.x, by = me.y - K1.y, bz = me.z - K1.z;
+                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
+                ks += fast_exp(sc.kde2 * fmaf(bx, bx, fmaf(by, by, bz * bz)));
+            }
+            if (k < kr.y) {
+                const float4 K0 = J.kde[k];
+                const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
+                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
+            }
+        }
+        if (active && (!mcs0 || slice == 0)) {
+            const int MS = mcs0 ? 1 : S;
+            const float4 *lig2 = J.lig2 + (size_t)idx_lig * NL;
+            for (int row = a * MS + (mcs0 ? 0 : slice); row < J.nrow; row += MS * NL) {
+                const int rb = J.mrow[row], re = J.mrow[row + 1];
+                float s = 0.f;
+#pragma unroll 4
+                for (int k2 = rb; k2 < re; ++k2) {
+                    const float4 E = J.ment[k2];
+                    const float4 X = lig2[__float_as_int(E.w)];
+                    const float ax = r00 * X.x + r01 * X.y + r02 * X.z + tx;
+                    const float ay = r10 * X.x + r11 * X.y + r12 * X.z + ty;
+                    const float az = r20 * X.x + r21 * X.y + r22 * X.z + tz;
+                    const float dx = ax - E.x, dy = ay - E.y, dz = az - E.z;
+                    s += fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                }
+                mcs += J.mtcc[row] * sqrtf(s / (float)(re - rb));
+            }
+        }
+
         // ---- pair terms over this slice's points ----
         float evdw = 0.f, eele = 0.f, epmf = 0.f, epsp = 0.f, ehdb = 0.f, ehpc = 0.f;
         float cvdw = 0.f, cele = 0.f, cpmf = 0.f, cpsp = 0.f, chdb = 0.f, chpc = 0.f;
@@ -517,7 +570,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         const int kc = __builtin_amdgcn_readfirstlane(kpos > (float)(kCenters - 1) ? 0 : (int)fmaxf(kpos, 0.0f));
         const float cc = (float)kc * tg.center_dr * 0.57735026918962576f;
         const float4 *__restrict__ ptc = tg.pt + ((size_t)kc * tg.nconf + prt) * tg.npad;
-        const float2 *__restrict__ pak = tg.pa + (size_t)kc * tg.npad;
+        const float *__restrict__ pak = tg.pw + (size_t)kc * tg.npad;
         const float ux = me.x - cc, uy = me.y - cc, uz = me.z - cc;
         const float r2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
         const float rho_out = sqrtf(wave_max(r2)), rho_in = sqrtf(wave_min(r2));
@@ -657,50 +710,6 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         }
         } // general path
 
-        // KDE over the atom type's run of the type-sorted table
-        // (energy_kde_v1.cpp:1-77): slice s takes points kr.x + s, kr.x + s + S, ...
-        // MCS rows over the file-order atom set (energy_mcs_v3_ell.cpp):
-        // worker a * S + s of the replica takes rows a * S + s, + S * N_L, ...,
-        // so the rows of a small table still fall on every slice
-        float ks = 0.f, mcs = 0.f;
-        const int2 kr = J.krange[a];
-        const bool kde0 = task.layout & LAYOUT_KDE_SLICE0, mcs0 = task.layout & LAYOUT_MCS_SLICE0;
-        if (active && (!kde0 || slice == 0)) {
-            const int KS = kde0 ? 1 : S;
-            int k = kr.x + (kde0 ? 0 : slice);
-            for (; k + KS < kr.y; k += 2 * KS) {
-                const float4 K0 = J.kde[k], K1 = J.kde[k + KS];
-                const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
-                const float bx = me.x - K1.x, by = me.y - K1.y, bz = me.z - K1.z;
-                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
-                ks += fast_exp(sc.kde2 * fmaf(bx, bx, fmaf(by, by, bz * bz)));
-            }
-            if (k < kr.y) {
-                const float4 K0 = J.kde[k];
-                const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
-                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
-            }
-        }
-        if (active && (!mcs0 || slice == 0)) {
-            const int MS = mcs0 ? 1 : S;
-            const float4 *lig2 = J.lig2 + (size_t)idx_lig * NL;
-            for (int row = a * MS + (mcs0 ? 0 : slice); row < J.nrow; row += MS * NL) {
-                const int rb = J.mrow[row], re = J.mrow[row + 1];
-                float s = 0.f;
-#pragma unroll 4
-                for (int k2 = rb; k2 < re; ++k2) {
-                    const float4 E = J.ment[k2];
-                    const float4 X = lig2[__float_as_int(E.w)];
-                    const float ax = r00 * X.x + r01 * X.y + r02 * X.z + tx;
-                    const float ay = r10 * X.x + r11 * X.y + r12 * X.z + ty;
-                    const float az = r20 * X.x + r21 * X.y + r22 * X.z + tz;
-                    const float dx = ax - E.x, dy = ay - E.y, dz = az - E.z;
-                    s += fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                }
-                mcs += J.mtcc[row] * sqrtf(s / (float)(re - rb));
-            }
-        }
-
         // ---- cross-slice sum ----
         if (S > 1 && slice > 0) {
             float *r = red + ((size_t)(slice - 1) * SW * 64 + slot) * kRed;
@@ -836,7 +845,11 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             for (int k = 0; k < 6; ++k) rp.movematrix[k] = st[i * 16 + k];
             for (int k = 0; k < GD_MAXWEI; ++k) rp.energy[k] = st[i * 16 + 6 + k];
             rp.step = ist[i * kIst + 2];
-            rp.is_accept = ist[i * kIst + 1];
+            // the reference sets is_accept once, after the chunk's last step
+            // (kernel_cpp_montecarlo_soa.cpp:415), and the record entries of
+            // the chunk carry the value from before it: earlier step slices
+            // leave it in place for the later slices' records
+            if (kslice == nslice - 1) rp.is_accept = ist[i * kIst + 1];
             if (RECORDS) J.next_entry[task.rep0 + i] = ist[i * kIst + 0];
             J.mc_acc[task.rep0 + i] += ist[i * kIst + 4]; // this workgroup owns the replica
         }

@@ -62,6 +62,13 @@ class Device:
         L.check(self.lib.gd_job_shape(self.h, job, s))
         return tuple(s)
 
+    def batch_plan(self):
+        """(tasks, workgroups launched, step slices per task) of the last
+        persistent launch."""
+        s = (C.c_int * 3)()
+        L.check(self.lib.gd_batch_plan(self.h, s))
+        return tuple(s)
+
     def free_job(self, job):
         L.check(self.lib.gd_free_job(self.h, job))
 

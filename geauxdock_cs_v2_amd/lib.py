@@ -94,6 +94,7 @@ def _declare_device(lib):
     _sig(lib, "gd_server_poll", i32, [vp, i32, C.POINTER(D.RequestResult), C.POINTER(D.ReplicaMC)])
     _sig(lib, "gd_set_stream", i32, [vp, i32, u64, C.POINTER(u32), i32])
     _sig(lib, "gd_job_shape", i32, [vp, i32, pi])
+    _sig(lib, "gd_batch_plan", i32, [vp, pi])
     _sig(lib, "gd_read_mclog", i32, [vp, i32, pi, pi])
     _sig(lib, "gd_server_cancel", i32, [vp, i32])
     _sig(lib, "gd_server_drain", i32, [vp])

@@ -191,6 +191,12 @@ int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opt
  * reference, whose block shape is a build constant, TperB): shape[0..3] =
  * waves per workgroup, slot waves, point slices, replicas per workgroup. */
 int gd_job_shape(gd_ctx *ctx, int job_id, int *shape);
+/* Work queue of the context's last persistent Monte Carlo launch (no
+ * counterpart in the reference, whose grid-stride loop over replicas,
+ * src/backend_gpu/kernel_cuda_l1_montecarlo.cu:45, has no queue):
+ * plan[0..2] = tasks, workgroups launched (the resident count, at most one
+ * per item), step slices per task. */
+int gd_batch_plan(gd_ctx *ctx, int *plan);
 
 /* ------------------------------------------------------------------ */
 /* serving: continuous batching of docking requests on one GPU          */
