@@ -15,12 +15,73 @@ namespace gd {
 #define GD_HIP(call)                                                                                 \
     do {                                                                                             \
         hipError_t e_ = (call);                                                                      \
-        if (e_ != hipSuccess) return fail(GD_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(e_ == hipErrorOutOfMemory ? GD_ERR_NOMEM : GD_ERR_DEVICE,                     \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                          \
     } while (0)
+
+Arena::~Arena()
+{
+    if (base) (void)hipFree(base);
+}
+
+hipError_t Arena::init(size_t n)
+{
+    cap = rounded(n);
+    hipError_t e = hipMalloc(&base, cap);
+    if (e != hipSuccess) {
+        base = nullptr;
+        cap = 0;
+        return e;
+    }
+    free_blocks.clear();
+    free_blocks[0] = cap;
+    in_use = 0;
+    return hipSuccess;
+}
+
+void *Arena::take(size_t n)
+{
+    n = rounded(n);
+    for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+        if (it->second < n) continue;
+        const size_t off = it->first, size = it->second;
+        free_blocks.erase(it);
+        if (size > n) free_blocks[off + n] = size - n;
+        in_use += n;
+        return (char *)base + off;
+    }
+    return nullptr;
+}
+
+void Arena::give(void *q, size_t n)
+{
+    n = rounded(n);
+    size_t off = (size_t)((char *)q - (char *)base);
+    in_use -= n;
+    auto next = free_blocks.lower_bound(off);
+    if (next != free_blocks.end() && off + n == next->first) { // merge with the block after
+        n += next->second;
+        next = free_blocks.erase(next);
+    }
+    if (next != free_blocks.begin()) { // merge with the block before
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+            prev->second += n;
+            return;
+        }
+    }
+    free_blocks[off] = n;
+}
 
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p) {
+        if (arena)
+            arena->give(p, bytes);
+        else
+            (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
 }
@@ -29,6 +90,12 @@ hipError_t DevBuf::alloc(size_t n)
 {
     release();
     if (n == 0) n = 16;
+    if (arena) {
+        p = arena->take(n);
+        if (!p) return hipErrorOutOfMemory;
+        bytes = n;
+        return hipSuccess;
+    }
     hipError_t e = hipMalloc(&p, n);
     if (e == hipSuccess) bytes = n;
     else p = nullptr;
@@ -319,6 +386,16 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     // at radius >= rho' (a ligand that has left the pocket): ladder_in[seg][k]
     // ends the segment's leading run of points with largest radius <= rung k.
     double far_d = std::max(9.0, 1.0 / (double)ep.el1);
+    // Mid field: beyond mid_d the hydrophobic term is exactly 0 (u clamps to
+    // 0 past 9 A), no pair is inside the electrostatic core and the H-bond
+    // Gaussian is below e^-30 of hdb1; only LJ, 1/d electrostatics and the
+    // contact sigmoid vary.
+    double mid_d = far_d;
+    for (int tp = 0; tp < GD_MAXTP1; ++tp)
+        for (int tl = 0; tl < GD_MAXTP2; ++tl)
+            if (ep.hdb0[tl][tp] > 0.1f && ep.hdb1[tl][tp] != 0.0f)
+                mid_d = std::max(mid_d, (double)ep.hdb0[tl][tp] + std::sqrt(60.0) / std::fabs((double)ep.hdb1[tl][tp]));
+    mid_d += 0.05;
     {
         const double tp_ = 26.0 * std::log(2.0);
         for (int tp = 0; tp < GD_MAXTP1; ++tp)
@@ -468,6 +545,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.ladder = t.ladder.as<int>();
     t.dt.ladder_in = t.ladder_in.as<int>();
     t.dt.far_d = (float)far_d;
+    t.dt.mid_d = (float)std::min(mid_d, far_d);
     t.dt.vfar_d = (float)std::min(vfar_d, 1e30);
     t.dt.lean_e_d = (float)std::min(lean_e_d, 1e30);
     t.dt.lean_soft_d = (float)std::min(lean_soft_d, 1e30);
@@ -544,7 +622,7 @@ size_t gd_job_footprint(int lig_natom, int n_lig, int kde_npoint, int mcs_entrie
            sizeof(int) * (mcs_nrow + 1) + sizeof(float4) * mcs_entries + sizeof(float) * mcs_nrow +
            sizeof(float) * n_tmp + sizeof(gd_ReplicaMC) * (size_t)n_rep * (1 + record_cap) +
            3 * sizeof(int) * (size_t)n_rep /* next_entry, McLog counters */ + sizeof(DevJob) +
-           12 * 256 // allocation granularity slack
+           17 * 256 // allocation granularity slack (one block per table)
            + 64 * 1024; // PSP point rows (100 bytes per point, up to ~650 points)
 }
 
@@ -634,6 +712,10 @@ int gd_upload_job(gd_ctx *ctx, int target_id, const gd_Complex *cx, uint64_t see
 
     const int id = ctx->next_id++;
     JobRes &j = ctx->jobs[id];
+    if (ctx->job_arena)
+        for (DevBuf *b : {&j.lig3, &j.lig2, &j.lt, &j.krange, &j.kde, &j.mrow, &j.ment, &j.mtcc, &j.mbeta, &j.psp_pt,
+                          &j.psp_row, &j.rep, &j.rec, &j.next_entry, &j.self, &j.kstream, &j.mclog})
+            b->arena = ctx->job_arena;
     // a failed upload leaves no job behind (the device buffers free with it)
     struct Drop {
         gd_ctx *ctx;

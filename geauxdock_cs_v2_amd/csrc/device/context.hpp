@@ -2,6 +2,7 @@
 // host mirrors the launchers need.
 #pragma once
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -13,20 +14,40 @@ namespace gd {
 
 int fail(int code, const std::string &msg);
 
-// owning device buffer
+// A device-memory arena: one allocation carved into 256-byte-aligned blocks
+// by first fit, with neighbouring free blocks merged.  Taking and giving back
+// blocks calls no HIP function, so a thread that owns running streams can
+// release memory without the device-wide wait of hipFree.
+struct Arena {
+    void *base = nullptr;
+    size_t cap = 0, in_use = 0;
+    std::map<size_t, size_t> free_blocks; // offset -> size
+    Arena() = default;
+    Arena(const Arena &) = delete;
+    Arena &operator=(const Arena &) = delete;
+    ~Arena();
+    hipError_t init(size_t bytes);
+    void *take(size_t n); // null when no block is large enough
+    void give(void *p, size_t n);
+    static size_t rounded(size_t n) { return (std::max<size_t>(n, 1) + 255) & ~(size_t)255; }
+};
+
+// owning device buffer, from hipMalloc or from an arena
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    Arena *arena = nullptr; // set before alloc: blocks come from (and return to) it
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), arena(o.arena) { o.p = nullptr; o.bytes = 0; }
     DevBuf &operator=(DevBuf &&o) noexcept
     {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            arena = o.arena;
             o.p = nullptr;
             o.bytes = 0;
         }
@@ -98,6 +119,9 @@ struct gd_ctx {
     gd::DevBuf queue, ladders, bases;
     // work queue of the last persistent launch: tasks, grid, step slices
     int last_plan[3] = {0, 0, 0};
+    // when set (by a server, around its own uploads), job uploads take their
+    // device tables from this arena instead of hipMalloc
+    gd::Arena *job_arena = nullptr;
     // streams of the batched form's workgroup groups (created on first use,
     // default priority)
     static constexpr int kGroups = 2;

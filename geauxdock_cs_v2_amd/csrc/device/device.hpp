@@ -92,6 +92,7 @@ struct DevTarget {
     const int4 *seg;   // [nseg] {point type, begin, end, hydrophobic flag}
     const int *ladder; // [kCenters][nseg][kFarRungs] first point of the segment at distance >= rung from the centre
     const int *ladder_in; // [kCenters][nseg][kFarRungs] end of the segment's leading points with largest distance <= rung
+    float mid_d;       // distance beyond which the hydrophobic, core and H-bond terms vanish (mid_d <= far_d)
     float far_d;       // distance beyond which only LJ and 1/d electrostatics vary
     float vfar_d;      // distance beyond which 1 / (1 + e) of the soft-core LJ is 1 - e + e^2 to float rounding
     float lean_e_d;    // distance beyond which it is 1 - e (e <= 2^-12.5)

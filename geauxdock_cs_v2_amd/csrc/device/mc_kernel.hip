@@ -178,7 +178,7 @@ __device__ __forceinline__ void pair_terms(float d2, float rinv, float hw, const
 // groups of four with one wave-wide test for electrostatic-core pairs
 // (closer than 1 / el1: clashes, rare), whose corrections are skipped
 // otherwise.
-template <bool HPC, bool HDB>
+template <bool HPC, bool HDB, bool CORE = true>
 __device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_, int pb, int pe, const Slot &me,
                                            const PairParams &q, const DevScalars &sc, Sums &s)
 {
@@ -207,10 +207,12 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_,
             ge = fmaf(P[j].w, rinv[j], ge);
         }
         // core corrections where some lane of the wave has a core pair in the group
-        const bool near = (d2[0] < sc.el_dc2) | (d2[1] < sc.el_dc2) | (d2[2] < sc.el_dc2) | (d2[3] < sc.el_dc2);
-        if (__any(near)) {
+        if (CORE) {
+            const bool near = (d2[0] < sc.el_dc2) | (d2[1] < sc.el_dc2) | (d2[2] < sc.el_dc2) | (d2[3] < sc.el_dc2);
+            if (__any(near)) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) ge += ele_core_delta(d2[j], rinv[j], P[j].w, sc);
+                for (int j = 0; j < 4; ++j) ge += ele_core_delta(d2[j], rinv[j], P[j].w, sc);
+            }
         }
         kahan_add(s.e, s.ec, ge);
     }
@@ -220,7 +222,7 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_,
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float rinv = __builtin_amdgcn_rsqf(d2);
         pair_terms<HPC, HDB>(d2, rinv, HPC ? pw[p] : 0.0f, q, sc, s);
-        kahan_add(s.e, s.ec, fmaf(P4.w, rinv, ele_core_delta(d2, rinv, P4.w, sc)));
+        kahan_add(s.e, s.ec, fmaf(P4.w, rinv, CORE ? ele_core_delta(d2, rinv, P4.w, sc) : 0.0f));
     }
 }
 
@@ -575,6 +577,10 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         const float r2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
         const float rho_out = sqrtf(wave_max(r2)), rho_in = sqrtf(wave_min(r2));
         const int kf = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho_out + tg.far_d - kFarR0) * (1.0f / kFarDr))));
+        // the same rungs for the mid field (mid_d <= far_d, so km <= kf and kmi >= ki)
+        const int km = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho_out + tg.mid_d - kFarR0) * (1.0f / kFarDr))));
+        const int kmi = __builtin_amdgcn_readfirstlane(
+            min(kFarRungs - 1, (int)floorf((rho_in - tg.mid_d - kFarR0) * (1.0f / kFarDr))));
         const int ki = __builtin_amdgcn_readfirstlane(
             min(kFarRungs - 1, (int)floorf((rho_in - tg.far_d - kFarR0) * (1.0f / kFarDr))));
         // the same rungs for the very far field (vfar_d >= far_d, so kv >= kf
@@ -635,12 +641,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             const int ob = kf < kFarRungs ? ladder[si * kFarRungs + kf] : sg.z;
             const int ive = kvi >= 0 ? ladder_in[si * kFarRungs + kvi] : sg.y;
             const int ovb = kv < kFarRungs ? ladder[si * kFarRungs + kv] : sg.z;
-            // very far [b, v1), far [v1, m1), near [m1, mid), far [mid, v2),
-            // very far [v2, e); the inner and outer runs cannot share a
-            // point (ive <= ie <= ob <= ovb), the clamps keep the order
+            const int imi = kmi >= 0 ? ladder_in[si * kFarRungs + kmi] : sg.y;
+            const int om = km < kFarRungs ? ladder[si * kFarRungs + km] : sg.z;
+            // very far [b, v1), far [v1, m1), mid [m1, n1), near [n1, n2),
+            // mid [n2, mid), far [mid, v2), very far [v2, e); the inner and
+            // outer runs cannot share a point (ive <= ie <= imi <= om <= ob
+            // <= ovb), the clamps keep the order
             const int v1 = min(max(ive, b), e);
             const int m1 = min(max(ie, v1), e);
-            const int mid = min(max(ob, m1), e);
+            const int n1 = min(max(imi, m1), e);
+            const int n2 = min(max(om, n1), e);
+            const int mid = min(max(ob, n2), e);
             const int v2 = min(max(ovb, mid), e);
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
@@ -651,8 +662,8 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             // plain FP32 sums over blocks of at most kBlock points, block sums
             // added with compensation: the electrostatic sum cancels by up to
             // 1e4 over a pose, which long running FP32 sums cannot hold
-            for (int pb = m1; pb < mid; pb += kBlock) {
-                const int pe = min(pb + kBlock, mid);
+            for (int pb = n1; pb < n2; pb += kBlock) {
+                const int pe = min(pb + kBlock, n2);
                 Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (sg.w) { // segment flag: hydrophobic-contact points
                     if (hdb)
@@ -672,14 +683,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 acc_add<KAHAN>(ehdb, chdb, bs.h);
                 acc_add<KAHAN>(ehpc, chpc, bs.q);
             }
-            for (int part = 0; part < 4; ++part) {
-                // far [v1, m1), [mid, v2); very far [b, v1), [v2, e)
-                const int fb = part == 0 ? v1 : part == 1 ? mid : part == 2 ? b : v2;
-                const int fe = part == 0 ? m1 : part == 1 ? v2 : part == 2 ? v1 : e;
+            for (int part = 0; part < 6; ++part) {
+                // mid [m1, n1), [n2, mid); far [v1, m1), [mid, v2); very
+                // far [b, v1), [v2, e)
+                const int fb = part == 0 ? m1 : part == 1 ? n2 : part == 2 ? v1 : part == 3 ? mid : part == 4 ? b : v2;
+                const int fe = part == 0 ? n1 : part == 1 ? mid : part == 2 ? m1 : part == 3 ? v2 : part == 4 ? v1 : e;
                 for (int pb = fb; pb < fe; pb += kBlock) {
                     const int pe = min(pb + kBlock, fe);
                     Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                     if (part < 2)
+                        pair_block<false, false, false>(ptc, pak, pb, pe, me, q, sc, bs);
+                    else if (part < 4)
                         pair_block_far(ptc, pb, pe, me, q, sc, bs);
                     else
                         pair_block_vfar<0>(ptc, pb, pe, me, q, sc, bs);

@@ -5,12 +5,15 @@
 // requests in arrival order while the device-memory budget and the slot count
 // allow (admission by HBM footprint) and places each in one of two lanes.  A
 // lane advances all of its requests by one chunk of Monte Carlo steps in a
-// single fused launch on its own stream; the two lanes run side by side and
-// are relaunched independently, so the tail of one lane's launch (its last,
-// partly filled round of workgroups) overlaps the other lane's work.
-// Requests that arrive while the lanes are busy join an idle lane at its
-// next chunk boundary, each at its own step counter.  Finished requests are
-// read back, their device memory freed, and the freed room lets queued
+// single persistent launch on its own stream (the resident workgroups walk
+// the lane's cost-ordered task queue); the two lanes run side by side and
+// are relaunched independently, so when one lane's queue runs dry its
+// workgroups give way to the other lane's.  Requests that arrive while the
+// lanes are busy join an idle lane at its next chunk boundary, each at its
+// own step counter; a chunk ends at the request's next exchange boundary.
+// Finished requests are read back and their tables returned to the server's
+// device arena (no hipFree, whose device-wide wait would hold the worker
+// until the other lane's launch ends), and the freed room lets queued
 // requests in.
 //
 // Every request keeps the workgroup shape its ligand size alone decides and
@@ -94,11 +97,23 @@ struct Lane {
     std::vector<int> s2, exch; // steps of the chunk, exchange round after it
     std::vector<DevJob> table;
     std::vector<DevTask> tasks;
-    DevBuf jobs_dev, tasks_dev;
+    DevBuf jobs_dev, tasks_dev, queue; // queue: the persistent launch's counter and done[]
     hipEvent_t done = nullptr;
     hipError_t err = hipSuccess;
     bool busy = false;
 };
+
+// Copy to a device buffer that grows by doubling: a lane's tables are
+// replaced rarely, so the device-wide wait of the old buffer's hipFree is
+// rare too.
+hipError_t upload_grow(DevBuf &b, const void *src, size_t bytes, hipStream_t s)
+{
+    if (!b.p || b.bytes < bytes) {
+        const hipError_t e = b.alloc(std::max(bytes, 2 * b.bytes));
+        if (e != hipSuccess) return e;
+    }
+    return bytes ? hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+}
 
 struct Request {
     int id = 0;
@@ -137,6 +152,10 @@ struct gd_server {
     std::thread worker;
     std::chrono::steady_clock::time_point t0;
     Lane lanes[kLanes];
+    // request tables live in this arena (capacity: the HBM budget plus
+    // alignment slack), so finishing a request frees no device memory through
+    // hipFree while the other lane runs
+    Arena arena;
 
     double now() const
     {
@@ -197,7 +216,14 @@ void gd_server::admit_locked()
         if (used + q.footprint > opts.hbm_budget) break; // wait for room, keep arrival order
         pending.pop_front();
         int jid = -1;
+        ctx->job_arena = &arena; // this upload's tables come from the server's arena
         const int rc = gd_upload_job(ctx, target, &q.job.s.view, q.seed, &jid);
+        ctx->job_arena = nullptr;
+        if (rc == GD_ERR_NOMEM && !active.empty()) {
+            // the arena is fragmented: wait for a running request to finish
+            pending.push_front(q.id);
+            break;
+        }
         if (rc != GD_OK) {
             finish_locked(q, rc, std::string("admission: ") + gd_last_error());
             continue;
@@ -254,7 +280,10 @@ void gd_server::prepare_locked(int k)
         Request &q = reqs.at(id);
         if (q.lane != k) continue;
         JobRes &j = ctx->jobs.at(q.job_id);
-        const int s2 = std::min(opts.chunk_steps, q.steps - q.done);
+        int s2 = std::min(opts.chunk_steps, q.steps - q.done);
+        // a chunk ends at the request's next exchange boundary, so every
+        // round falls due whatever the chunk length
+        if (opts.exchange_interval > 0) s2 = std::min(s2, opts.exchange_interval - q.done % opts.exchange_interval);
         const int after = q.done + s2;
         L.ids.push_back(id);
         L.s2.push_back(s2);
@@ -275,14 +304,28 @@ void gd_server::launch(int k)
 {
     Lane &L = lanes[k];
     hipStream_t gs = ctx->group_stream[k];
-    int max_rep = 1;
-    for (const DevTask &t : L.tasks) max_rep = std::max(max_rep, t.nrep);
+    int max_rep = 1, min_s2 = 1 << 30;
+    for (const DevTask &t : L.tasks) {
+        max_rep = std::max(max_rep, t.nrep);
+        min_s2 = std::min(min_s2, t.s2max);
+    }
     const TargetRes &t = ctx->targets.at(target);
-    hipError_t e = upload(L.jobs_dev, L.table.data(), sizeof(DevJob) * L.table.size(), gs);
-    if (e == hipSuccess) e = upload(L.tasks_dev, L.tasks.data(), sizeof(DevTask) * L.tasks.size(), gs);
-    if (e == hipSuccess)
-        e = launch_mc(t.dt, t.sc, L.jobs_dev.as<DevJob>(), L.tasks_dev.as<DevTask>(), (int)L.tasks.size(), kServerWaves,
-                      max_rep, true, false, gs, nullptr);
+    const int ntask = (int)L.tasks.size();
+    hipError_t e = upload_grow(L.jobs_dev, L.table.data(), sizeof(DevJob) * L.table.size(), gs);
+    if (e == hipSuccess) e = upload_grow(L.tasks_dev, L.tasks.data(), sizeof(DevTask) * L.tasks.size(), gs);
+    if (e == hipSuccess && L.queue.bytes < sizeof(int) * ((size_t)ntask + 1))
+        e = L.queue.alloc(std::max(sizeof(int) * ((size_t)ntask + 1), 2 * L.queue.bytes));
+    if (e == hipSuccess) {
+        // a persistent launch over the lane's tasks: the lanes' launches
+        // share the resident workgroups, the second one filling in as the
+        // first one's queue runs dry
+        const int slots = resident_slots(ctx, kServerWaves, max_rep);
+        DevQueue qd{L.queue.as<int>(), L.queue.as<int>() + 1, ntask, choose_slices(ntask, slots, min_s2)};
+        e = launch_queue_reset(qd, gs);
+        if (e == hipSuccess)
+            e = launch_mc_queue(t.dt, t.sc, L.jobs_dev.as<DevJob>(), L.tasks_dev.as<DevTask>(), qd, slots, kServerWaves,
+                                max_rep, true, false, gs, nullptr);
+    }
     for (size_t i = 0; i < L.ids.size() && e == hipSuccess; ++i) {
         if (!L.exch[i]) continue;
         int rounds, n_lig, n_prt, n_tmp;
@@ -410,6 +453,12 @@ int gd_server_create(gd_ctx *ctx, int target_id, const gd_server_opts *opts, gd_
     s->ctx = ctx;
     s->target = target_id;
     s->opts = *opts;
+    // the budget plus each admitted request's alignment slack
+    const size_t slack = (size_t)opts->max_active_jobs * 64 * 1024;
+    if (s->arena.init(opts->hbm_budget + slack) != hipSuccess) {
+        delete s;
+        return fail(GD_ERR_NOMEM, "cannot reserve the serving budget in device memory");
+    }
     s->t0 = std::chrono::steady_clock::now();
     s->worker = std::thread([s] { s->loop(); });
     *out = s;
@@ -427,7 +476,7 @@ void gd_server_destroy(gd_server *srv)
     if (srv->worker.joinable()) srv->worker.join();
     for (auto &kv : srv->reqs)
         if (kv.second.job_id >= 0) gd_free_job(srv->ctx, kv.second.job_id);
-    delete srv;
+    delete srv; // every job from its arena is gone
 }
 
 int gd_server_submit(gd_server *srv, const gd_Complex *cx, int steps, uint64_t seed, int *req_id)

@@ -73,6 +73,37 @@ def accept_uniform(lib, seed, idx_rep, step):
     return float(np.float32(lib.gd_stream_k(seed, 0, idx_rep, step, 6)) * np.float32(2.0 ** -31))
 
 
+def shifted_e64(cx, r, move):
+    """FP64 energies of replica r at `move` translated by 1e-4 A along +-x,
+    +-y, +-z: (6, 10)."""
+    mv = np.repeat(np.asarray(move, np.float64)[None, :6], 6, 0)
+    mv[:, :3] += np.concatenate([np.eye(3), -np.eye(3)]) * 1e-4
+    return O.energy(cx.view, np.full(6, r, np.int32), mv.astype(np.float32), "f64")
+
+
+PSP_TERMS = {3, 9}  # the pocket-specific term and the total that carries it
+
+
+def psp_step_ties(cx, rep_idx, moves, eg, e64):
+    """Entries whose only disagreement with FP64 is in the pocket-specific
+    term (and the total): that term adds a table entry iff an atom-point
+    distance is within pmf0 (energy_prt_v1.cpp:107-109), a step function that
+    flips when the distance sits at pmf0 within float rounding.  Such an entry
+    passes when FP64 at a pose translated by 1e-4 A along an axis gives the
+    GPU's values within REL_TOL.  Returns the entries that stay unexplained."""
+    errs = rel_err(eg, e64)
+    bad = [b for b in range(len(eg)) if errs[b].max() > REL_TOL]
+    left = []
+    for b in bad:
+        if not set(np.nonzero(errs[b] > REL_TOL)[0]) <= PSP_TERMS:
+            left.append(b)
+            continue
+        ep = shifted_e64(cx, rep_idx[b], moves[b])
+        if not np.any(rel_err(np.repeat(eg[b][None, :], 6, 0), ep).max(1) <= REL_TOL):
+            left.append(b)
+    return bad, left
+
+
 def compare_step(lib, cx, seed, step, before, g, o):
     """One MC step taken by the GPU (g) and by the oracle (o) from the same
     state `before`.  Proposals are rebuilt from the stream and must be
@@ -107,15 +138,21 @@ def compare_step(lib, cx, seed, step, before, g, o):
             tol_g = GPU_WINDOW * scale_t + 1e-7
             tol_c = CPU_WINDOW * scale_t + 1e-7
         ga, oa = bool(g[r].is_accept), bool(o[r].is_accept)
-        if ga != acc64:
-            assert abs(u - thr) <= tol_g, ("GPU decision off the FP64 threshold", step, r, u, thr, tol_g)
+        if ga != acc64 and abs(u - thr) > tol_g:
+            # a pocket-specific step tie (see psp_step_ties): the decision
+            # must be the FP64 one at a pose within 1e-4 A
+            ep = shifted_e64(cx, r, prop[r])
+            thr_p = np.exp((ep[:, 9] - float(before[r].energy[9])) * mb)
+            assert np.any((u < thr_p) == ga), ("GPU decision off the FP64 threshold", step, r, u, thr, tol_g)
         if oa != acc64:
             assert abs(u - thr) <= tol_c, ("oracle decision off the FP64 threshold", step, r, u, thr, tol_c)
         flips += ga != oa
         if ga:
             assert np.array_equal(np.array(g[r].movematrix[:6], np.float32), prop[r]), (step, r)
-            err = rel_err(np.array(g[r].energy[:10]), e64[r]).max()
-            assert err <= REL_TOL, (step, r, err)
+            eg_r = np.array(g[r].energy[:10], np.float64)
+            if rel_err(eg_r, e64[r]).max() > REL_TOL:
+                _, left = psp_step_ties(cx, np.array([r], np.int32), prop[r:r + 1], eg_r[None], e64[r:r + 1])
+                assert not left, (step, r, rel_err(eg_r, e64[r]))
             assert g[r].step == step
         else:  # rejected: state unchanged
             assert list(g[r].movematrix) == list(before[r].movematrix)
@@ -232,10 +269,19 @@ def test_montecarlo_long_chunk_records(lib, ctx, kat_complex11):
     idx = np.array([r for r, _ in reps], np.int32)
     e64 = O.energy(cx.view, idx, np.array(moves), "f64")
     eg = np.array([list(rec[r * 100 + k].energy)[:10] for r, k in reps])
-    err = rel_err(eg, e64).max()
+    errs = rel_err(eg, e64)
+    bad, left = psp_step_ties(cx, idx, np.array(moves), eg, e64)
+    ok = np.ones(len(eg), bool)
+    ok[bad] = False
     say(f"record entries compared: {checked} over {n} replicas, {diverged} left the oracle's trajectory; "
-        f"max rel err vs FP64 {err:.2e}")
-    assert err <= REL_TOL
+        f"max rel err vs FP64 {errs[ok].max():.2e} ({len(bad)} pocket-specific step ties, "
+        f"{len(bad) - len(left)} of them reproduced within 1e-4 A)")
+    for b in left[:5]:
+        r, k = reps[b]
+        say(f"replica {r} entry {k} step {rec[r * 100 + k].step} of {ne[r]}: term {errs[b].argmax()}, "
+            f"GPU {np.array2string(eg[b], precision=5)}, FP64 {np.array2string(e64[b], precision=5)}")
+    assert not left
+    assert len(bad) <= max(2, checked // 20000)
     assert diverged <= max(2, n // 200)
 
 

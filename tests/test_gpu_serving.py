@@ -141,3 +141,21 @@ def test_cancel_and_bad_requests_fail_alone(setup):
     assert ra.status == 0
     assert rb.status == L.GD_ERR_STATE and "cancelled" in why_b, why_b
     assert rm.status == L.GD_ERR_ARG and "permutation" in why_m, why_m
+
+
+def test_exchange_rounds_fall_due_whatever_the_chunk(setup):
+    """A chunk ends at the request's next exchange boundary: with exchanges
+    every 50 steps, chunks of 30 steps (cut at 30, 50, 80, 100, ...) give
+    the same final replicas, temperature labels included, as chunks of 50;
+    with no exchange the labels stay where they started."""
+    lib, h, t, cx = setup
+    say("case: serving chunk 30 / exchange 50 against chunk 50 / exchange 50")
+    a = serve(lib, h, t, [(cx["B"], 230, 21)], chunk=30, exchange=50)[0]
+    b = serve(lib, h, t, [(cx["B"], 230, 21)], chunk=50, exchange=50)[0]
+    c = serve(lib, h, t, [(cx["B"], 230, 21)], chunk=30, exchange=0)[0]
+    assert a[0] == b[0] == c[0] == 0
+    assert a[1] == b[1]
+    start = cx["B"].replicas()
+    labels = lambda reps: [r.idx_tmp for r in reps]
+    assert labels(c[3]) == labels(start)
+    assert labels(a[3]) != labels(start)  # four rounds at 50, 100, 150, 200 moved some
