@@ -60,7 +60,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "replica-MC pose evaluations/s on 1 MI355X vs plain; p50/p99 job latency"
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "mc_kernel_stress_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "mc_kernel_stress_counters.md")
 
 
 def log(*a):
