@@ -200,12 +200,13 @@ int ensure_queue(gd_ctx *ctx, int ntask, DevQueue &q)
 }
 
 // One persistent launch of `tasks_dev` (ntask tasks, shape waves/max_rep,
-// slices chosen for s2max steps): queue reset, Monte Carlo kernel.
-int enqueue_mc(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const DevTask *tasks_dev, int ntask,
-               int s2max, int waves, int max_rep, bool comp, bool records, hipStream_t s, const int *step_base)
+// slices chosen for s2max steps) over the work queue q (room for ntask
+// tasks): queue reset, Monte Carlo kernel.
+int enqueue_mc_on(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const DevTask *tasks_dev, int ntask,
+                  int s2max, int waves, int max_rep, bool comp, bool records, hipStream_t s, const int *step_base,
+                  DevQueue q)
 {
-    DevQueue q;
-    if (int rc = ensure_queue(ctx, ntask, q)) return rc;
+    q.ntask = ntask;
     const int slots = resident_slots(ctx, waves, max_rep);
     q.nslice = choose_slices(ntask, slots, s2max);
     ctx->last_plan[0] = ntask;
@@ -214,6 +215,15 @@ int enqueue_mc(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const De
     GD_HIP(launch_queue_reset(q, s));
     GD_HIP(launch_mc_queue(t.dt, t.sc, jobs_dev, tasks_dev, q, slots, waves, max_rep, comp, records, s, step_base));
     return GD_OK;
+}
+
+// the same over the context's own work queue, grown to ntask tasks
+int enqueue_mc(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const DevTask *tasks_dev, int ntask,
+               int s2max, int waves, int max_rep, bool comp, bool records, hipStream_t s, const int *step_base)
+{
+    DevQueue q;
+    if (int rc = ensure_queue(ctx, ntask, q)) return rc;
+    return enqueue_mc_on(ctx, t, jobs_dev, tasks_dev, ntask, s2max, waves, max_rep, comp, records, s, step_base, q);
 }
 
 // Time `launches` consecutive persistent launches of `steps` steps of the
@@ -518,14 +528,16 @@ extern "C" int gd_batch_plan(gd_ctx *ctx, int *plan)
     return GD_OK;
 }
 
-extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
-                            double *elapsed_ms)
+namespace {
+
+// the batch's jobs (listed once each, one target) and its step checks
+int batch_jobs(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts, std::vector<JobRes *> &js,
+               int &target)
 {
     if (!ctx || !job_ids || n_jobs <= 0 || !opts) return fail(GD_ERR_ARG, "null or empty argument");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(GD_ERR_DEVICE, "hipSetDevice failed");
     if (opts->steps <= 0 || opts->exchange_interval < 0) return fail(GD_ERR_ARG, "bad step counts");
-    std::vector<JobRes *> js;
-    int target = -1;
+    target = -1;
     for (int i = 0; i < n_jobs; ++i) {
         auto it = ctx->jobs.find(job_ids[i]);
         if (it == ctx->jobs.end()) return fail(GD_ERR_STATE, "unknown job");
@@ -535,32 +547,35 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
         target = it->second.target;
         js.push_back(&it->second);
     }
-    const TargetRes &t = ctx->targets.at(target);
+    for (JobRes *j : js)
+        if (int rc = check_stream_range(*j, (long)j->steps_done + opts->steps)) return rc;
+    return GD_OK;
+}
+
+int batch_chunk(const gd_batch_opts *opts)
+{
+    return opts->exchange_interval > 0 ? opts->exchange_interval : std::min(opts->steps, GD_STEPS_PER_DUMP);
+}
+
+// Workgroup shapes: explicit in opts, the packing heuristic for a given wave
+// count, or (no wave count) autotuned when `tune`, else the autotuner's
+// cached choice where there is one and the heuristic at 12 waves otherwise.
+int batch_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js, const gd_batch_opts *opts,
+                 const DevJob *jobs_dev, bool tune, std::vector<Shape> &shapes)
+{
+    const int n_jobs = (int)js.size();
     const bool comp = opts->compensated != 0;
-    const int chunk = opts->exchange_interval > 0 ? opts->exchange_interval : std::min(opts->steps, GD_STEPS_PER_DUMP);
-    const bool records = opts->keep_records != 0;
-    hipStream_t s = ctx->stream;
-    std::vector<int> s1_of(n_jobs), round_of(n_jobs);
-    for (int i = 0; i < n_jobs; ++i) {
-        s1_of[i] = js[i]->steps_done;
-        round_of[i] = js[i]->exchange_rounds;
-        if (int rc = check_stream_range(*js[i], (long)s1_of[i] + opts->steps)) return rc;
-    }
-
-    // device job table for this batch
-    std::vector<DevJob> table(n_jobs);
-    for (int i = 0; i < n_jobs; ++i) {
-        if (records)
-            if (int rc = ensure_records(ctx, *js[i], chunk)) return rc;
-        table[i] = js[i]->hj;
-    }
-    DevBuf jobs_dev;
-    GD_HIP(upload(jobs_dev, table.data(), sizeof(DevJob) * n_jobs, s));
-
-    // workgroup shapes: explicit, or autotuned (per job, or per ligand-size
-    // bucket for several jobs)
     int W = opts->waves_per_group;
-    std::vector<Shape> shapes(n_jobs);
+    shapes.assign(n_jobs, Shape{});
+    auto heuristic = [&](int w) -> int {
+        if (w > 16) return fail(GD_ERR_ARG, "waves_per_group must be at most 16");
+        const int share = std::max(1, resident_slots(ctx, w, 8) / n_jobs);
+        for (int i = 0; i < n_jobs; ++i) {
+            if (js[i]->natom > 64 * w) return fail(GD_ERR_ARG, "ligand too large for the workgroup");
+            shapes[i] = heuristic_shape(js[i]->natom, w, js[i]->n_tmp * js[i]->n_lig, js[i]->n_prt, share);
+        }
+        return GD_OK;
+    };
     if (opts->slots_per_wave > 0 || opts->point_slices > 0) {
         if (W <= 0) return fail(GD_ERR_ARG, "explicit shapes need waves_per_group");
         if (W > 16 || (opts->point_slices > 0 && W % opts->point_slices))
@@ -572,59 +587,54 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
             if (opts->slots_per_wave > 0)
                 shapes[i].nrep = std::max(1, std::min(shapes[i].nrep, opts->slots_per_wave * SW / js[i]->natom));
         }
-    } else if (W > 0) {
-        if (W > 16) return fail(GD_ERR_ARG, "waves_per_group must be at most 16");
-        const int share = std::max(1, resident_slots(ctx, W, 8) / n_jobs);
-        for (int i = 0; i < n_jobs; ++i) {
-            if (js[i]->natom > 64 * W) return fail(GD_ERR_ARG, "ligand too large for the workgroup");
-            shapes[i] = heuristic_shape(js[i]->natom, W, js[i]->n_tmp * js[i]->n_lig, js[i]->n_prt, share);
-        }
-    } else if (n_jobs == 1) {
-        if (int rc = autotune_shape(ctx, t, *js[0], comp, opts->steps, shapes[0])) return rc;
-        W = shapes[0].waves;
-    } else {
-        if (int rc = autotune_batch(ctx, t, js, jobs_dev.as<DevJob>(), comp, opts->steps, shapes)) return rc;
-        W = shapes[0].waves;
+        return GD_OK;
     }
-    const int max_rep = max_rep_of(shapes);
-    for (int i = 0; i < n_jobs; ++i) js[i]->last_shape = shapes[i];
+    if (W > 0) return heuristic(W);
+    if (tune) {
+        if (n_jobs == 1) return autotune_shape(ctx, t, *js[0], comp, opts->steps, shapes[0]);
+        return autotune_batch(ctx, t, js, jobs_dev, comp, opts->steps, shapes);
+    }
+    if (int rc = heuristic(12)) return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (n_jobs == 1) {
+        auto it = g_tune_cache.find(std::make_tuple(js[0]->natom, t.npoint, js[0]->n_rep, js[0]->n_prt));
+        if (it != g_tune_cache.end()) shapes[0] = it->second;
+    }
+    return GD_OK;
+}
 
-    // tasks of a full chunk, and the exchange ladders of every job
-    std::vector<DevTask> tasks;
-    batch_tasks(js, shapes, s1_of, chunk, tasks);
-    const int ntask = (int)tasks.size();
-    DevBuf tasks_dev;
-    GD_HIP(upload(tasks_dev, tasks.data(), sizeof(DevTask) * ntask, s));
-    std::vector<int4> lad;
-    if (opts->exchange_interval > 0)
-        for (int i = 0; i < n_jobs; ++i)
-            if (js[i]->n_tmp > 1)
-                for (int L = 0; L < js[i]->n_lig * js[i]->n_prt; ++L) lad.push_back(make_int4(i, L, round_of[i], 0));
-    if (!lad.empty()) GD_HIP(upload(ctx->ladders, lad.data(), sizeof(int4) * lad.size(), s));
-    if (ctx->bases.bytes < 2 * sizeof(int)) GD_HIP(ctx->bases.alloc(2 * sizeof(int)));
-    int *step_base = ctx->bases.as<int>(), *round_base = ctx->bases.as<int>() + 1;
-    const int zero2[2] = {0, 0};
-    GD_HIP(hipMemcpyAsync(ctx->bases.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
-    DevQueue q0;
-    if (int rc = ensure_queue(ctx, ntask, q0)) return rc; // sized before any capture
+// Device memory a batched run uses beyond the jobs' own tables.
+struct BatchMem {
+    DevJob *jobs = nullptr;              // [n_jobs] job descriptors
+    DevTask *tasks = nullptr;            // [ntask] a full chunk's tasks
+    DevTask *tail = nullptr;             // [ntask] the last, shorter chunk's tasks
+    DevQueue queue{nullptr, nullptr, 0, 1}; // counter and done[ntask]
+    int4 *ladders = nullptr;             // [nlad] exchange ladders
+    int *bases = nullptr;                // [2] step and exchange-round counters
+};
 
-    // one chunk: queue reset, persistent Monte Carlo launch, exchange round
-    // over every ladder, step advance (no memset, so it can be captured)
+// The chunk sequence of a batched run, once its tables are in `mem`: per
+// full chunk a queue reset, the persistent Monte Carlo launch, an exchange
+// round over every ladder and the step advance (captured in a hipGraph when
+// use_graph), then the remaining steps.
+int batch_launches(gd_ctx *ctx, const TargetRes &t, const gd_batch_opts *opts, int chunk, int W, int max_rep,
+                   int ntask, int nlad, const BatchMem &mem, bool use_graph, double *elapsed_ms)
+{
+    hipStream_t s = ctx->stream;
+    const bool comp = opts->compensated != 0, records = opts->keep_records != 0;
     auto enqueue_period = [&]() -> int {
-        if (int rc = enqueue_mc(ctx, t, jobs_dev.as<DevJob>(), tasks_dev.as<DevTask>(), ntask, chunk, W, max_rep, comp,
-                                records, s, step_base))
+        if (int rc = enqueue_mc_on(ctx, t, mem.jobs, mem.tasks, ntask, chunk, W, max_rep, comp, records, s, mem.bases,
+                                   mem.queue))
             return rc;
-        if (!lad.empty())
-            GD_HIP(launch_exchange_list(jobs_dev.as<DevJob>(), ctx->ladders.as<int4>(), (int)lad.size(), s, round_base));
-        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, step_base, chunk, round_base);
+        if (nlad > 0) GD_HIP(launch_exchange_list(mem.jobs, mem.ladders, nlad, s, mem.bases + 1));
+        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, mem.bases, chunk, mem.bases + 1);
         GD_HIP(hipGetLastError());
         return GD_OK;
     };
-
     const int n_full = opts->steps / chunk;
     const int rest = opts->steps - n_full * chunk;
     Graph g;
-    if (n_full > 0 && opts->use_graph) {
+    if (n_full > 0 && use_graph) {
         GD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const int rc = enqueue_period();
         const hipError_t ce = hipStreamEndCapture(s, &g.graph);
@@ -632,34 +642,212 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
         GD_HIP(ce);
         GD_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     }
-    std::vector<DevTask> tail;
-    DevBuf tail_dev;
-    if (rest > 0) {
-        batch_tasks(js, shapes, s1_of, rest, tail);
-        GD_HIP(upload(tail_dev, tail.data(), sizeof(DevTask) * tail.size(), s));
-    }
+    const auto h0 = std::chrono::steady_clock::now();
     Event e0, e1;
-    GD_HIP(hipEventCreate(&e0.e));
-    GD_HIP(hipEventCreate(&e1.e));
-    GD_HIP(hipEventRecord(e0.e, s));
+    if (use_graph) {
+        GD_HIP(hipEventCreate(&e0.e));
+        GD_HIP(hipEventCreate(&e1.e));
+        GD_HIP(hipEventRecord(e0.e, s));
+    }
     for (int c = 0; c < n_full; ++c) {
-        if (opts->use_graph)
+        if (use_graph)
             GD_HIP(hipGraphLaunch(g.exec, s));
         else if (int rc = enqueue_period())
             return rc;
     }
     if (rest > 0)
-        if (int rc = enqueue_mc(ctx, t, jobs_dev.as<DevJob>(), tail_dev.as<DevTask>(), (int)tail.size(), rest, W,
-                                max_rep, comp, records, s, step_base))
+        if (int rc = enqueue_mc_on(ctx, t, mem.jobs, mem.tail, ntask, rest, W, max_rep, comp, records, s, mem.bases,
+                                   mem.queue))
             return rc;
-    GD_HIP(hipEventRecord(e1.e, s));
-    GD_HIP(hipEventSynchronize(e1.e));
     float ms = 0.f;
-    GD_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
-    if (elapsed_ms) *elapsed_ms = ms;
-    for (int i = 0; i < n_jobs; ++i) {
-        js[i]->steps_done += opts->steps;
-        if (opts->exchange_interval > 0) js[i]->exchange_rounds += n_full;
+    if (use_graph) {
+        GD_HIP(hipEventRecord(e1.e, s));
+        GD_HIP(hipEventSynchronize(e1.e));
+        GD_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+    } else {
+        GD_HIP(hipStreamSynchronize(s));
+        ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     }
+    if (elapsed_ms) *elapsed_ms = ms;
+    return GD_OK;
+}
+
+// host tables of a batched run: job descriptors, a full chunk's tasks, the
+// last chunk's tasks, exchange ladders
+void batch_tables(const std::vector<JobRes *> &js, const std::vector<Shape> &shapes, const gd_batch_opts *opts,
+                  int chunk, std::vector<DevJob> &table, std::vector<DevTask> &tasks, std::vector<DevTask> &tail,
+                  std::vector<int4> &lad)
+{
+    const int n_jobs = (int)js.size();
+    std::vector<int> s1_of(n_jobs);
+    table.resize(n_jobs);
+    for (int i = 0; i < n_jobs; ++i) {
+        s1_of[i] = js[i]->steps_done;
+        table[i] = js[i]->hj;
+    }
+    batch_tasks(js, shapes, s1_of, chunk, tasks);
+    const int rest = opts->steps % chunk;
+    tail.clear();
+    if (rest > 0) batch_tasks(js, shapes, s1_of, rest, tail);
+    lad.clear();
+    if (opts->exchange_interval > 0)
+        for (int i = 0; i < n_jobs; ++i)
+            if (js[i]->n_tmp > 1)
+                for (int L = 0; L < js[i]->n_lig * js[i]->n_prt; ++L)
+                    lad.push_back(make_int4(i, L, js[i]->exchange_rounds, 0));
+}
+
+void batch_done(const std::vector<JobRes *> &js, const std::vector<Shape> &shapes, const gd_batch_opts *opts,
+                int chunk)
+{
+    for (size_t i = 0; i < js.size(); ++i) {
+        js[i]->last_shape = shapes[i];
+        js[i]->steps_done += opts->steps;
+        if (opts->exchange_interval > 0) js[i]->exchange_rounds += opts->steps / chunk;
+    }
+}
+
+} // namespace
+
+extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
+                            double *elapsed_ms)
+{
+    std::vector<JobRes *> js;
+    int target = -1;
+    if (int rc = batch_jobs(ctx, job_ids, n_jobs, opts, js, target)) return rc;
+    const TargetRes &t = ctx->targets.at(target);
+    const int chunk = batch_chunk(opts);
+    hipStream_t s = ctx->stream;
+    if (opts->keep_records)
+        for (JobRes *j : js)
+            if (int rc = ensure_records(ctx, *j, chunk)) return rc;
+    std::vector<DevJob> table;
+    std::vector<DevTask> tasks, tail;
+    std::vector<int4> lad;
+    // the job table first: the multi-job autotuner launches from it
+    DevBuf jobs_dev;
+    {
+        std::vector<DevJob> t0(n_jobs);
+        for (int i = 0; i < n_jobs; ++i) t0[i] = js[i]->hj;
+        GD_HIP(upload(jobs_dev, t0.data(), sizeof(DevJob) * n_jobs, s));
+    }
+    std::vector<Shape> shapes;
+    if (int rc = batch_shapes(ctx, t, js, opts, jobs_dev.as<DevJob>(), true, shapes)) return rc;
+    const int W = shapes[0].waves, max_rep = max_rep_of(shapes);
+    batch_tables(js, shapes, opts, chunk, table, tasks, tail, lad);
+    const int ntask = (int)tasks.size();
+    DevBuf tasks_dev, tail_dev;
+    GD_HIP(upload(tasks_dev, tasks.data(), sizeof(DevTask) * ntask, s));
+    if (!tail.empty()) GD_HIP(upload(tail_dev, tail.data(), sizeof(DevTask) * tail.size(), s));
+    if (!lad.empty()) GD_HIP(upload(ctx->ladders, lad.data(), sizeof(int4) * lad.size(), s));
+    if (ctx->bases.bytes < 2 * sizeof(int)) GD_HIP(ctx->bases.alloc(2 * sizeof(int)));
+    const int zero2[2] = {0, 0};
+    GD_HIP(hipMemcpyAsync(ctx->bases.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+    BatchMem mem;
+    if (int rc = ensure_queue(ctx, ntask, mem.queue)) return rc; // sized before any capture
+    mem.jobs = jobs_dev.as<DevJob>();
+    mem.tasks = tasks_dev.as<DevTask>();
+    mem.tail = tail.empty() ? nullptr : tail_dev.as<DevTask>();
+    mem.ladders = ctx->ladders.as<int4>();
+    mem.bases = ctx->bases.as<int>();
+    if (int rc = batch_launches(ctx, t, opts, chunk, W, max_rep, ntask, (int)lad.size(), mem, opts->use_graph != 0,
+                                elapsed_ms))
+        return rc;
+    batch_done(js, shapes, opts, chunk);
+    return GD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The batched run without device allocation.  Its tables (job descriptors,
+// tasks, the work queue, exchange ladders, step and round counters) live in
+// a file-scope device array of kScratchBytes; each GPU holds its own copy
+// until the process ends, at the address hipGetSymbolAddress reports with
+// that device current (the array's name in host code is only a stand-in).
+// Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)8 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr; // device -> its copy of scratch
+
+// the device address of `device`'s scratch, looked up once per device
+int scratch_address(int device, void **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it == g_scratch_ptr.end()) {
+        void *p = nullptr;
+        GD_HIP(hipSetDevice(device));
+        GD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GD_ERR_DEVICE, "hipGetSymbolAddress returned a null scratch address");
+        it = g_scratch_ptr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return GD_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+} // namespace
+
+extern "C" size_t gd_batch_noalloc_capacity(void) { return kScratchBytes; }
+
+extern "C" int gd_batch_run_noalloc(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
+                                    double *elapsed_ms)
+{
+    std::vector<JobRes *> js;
+    int target = -1;
+    if (int rc = batch_jobs(ctx, job_ids, n_jobs, opts, js, target)) return rc;
+    const TargetRes &t = ctx->targets.at(target);
+    const int chunk = batch_chunk(opts);
+    if (opts->keep_records)
+        for (JobRes *j : js)
+            if (!j->rec.p || j->rec_cap < chunk)
+                return fail(GD_ERR_ARG, "keep_records needs each job's record window to hold a chunk already "
+                                        "(gd_batch_run or gd_montecarlo with records allocates it)");
+    // shapes without trial launches: explicit, the autotuner's cached choice
+    // or the heuristic
+    std::vector<Shape> shapes;
+    if (int rc = batch_shapes(ctx, t, js, opts, nullptr, false, shapes)) return rc;
+    const int W = shapes[0].waves, max_rep = max_rep_of(shapes);
+    std::vector<DevJob> table;
+    std::vector<DevTask> tasks, tail;
+    std::vector<int4> lad;
+    batch_tables(js, shapes, opts, chunk, table, tasks, tail, lad);
+    const size_t ntask = tasks.size();
+    // scratch layout: counters | queue | jobs | tasks | tail | ladders
+    const size_t o_bases = 0, o_queue = 256, o_jobs = o_queue + align256(sizeof(int) * (ntask + 1));
+    const size_t o_tasks = o_jobs + align256(sizeof(DevJob) * table.size());
+    const size_t o_tail = o_tasks + align256(sizeof(DevTask) * ntask);
+    const size_t o_lad = o_tail + align256(sizeof(DevTask) * tail.size());
+    const size_t need = o_lad + align256(sizeof(int4) * lad.size());
+    if (need > kScratchBytes)
+        return fail(GD_ERR_ARG, "the batch needs " + std::to_string(need) + " bytes of launch tables; the "
+                                "no-allocation entry point holds " + std::to_string(kScratchBytes));
+    void *base = nullptr;
+    if (int rc = scratch_address(ctx->device, &base)) return rc;
+    char *b = (char *)base;
+    hipStream_t s = ctx->stream;
+    const int zero2[2] = {0, 0};
+    GD_HIP(hipMemcpyAsync(b + o_bases, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+    GD_HIP(hipMemcpyAsync(b + o_jobs, table.data(), sizeof(DevJob) * table.size(), hipMemcpyHostToDevice, s));
+    GD_HIP(hipMemcpyAsync(b + o_tasks, tasks.data(), sizeof(DevTask) * ntask, hipMemcpyHostToDevice, s));
+    if (!tail.empty())
+        GD_HIP(hipMemcpyAsync(b + o_tail, tail.data(), sizeof(DevTask) * tail.size(), hipMemcpyHostToDevice, s));
+    if (!lad.empty()) GD_HIP(hipMemcpyAsync(b + o_lad, lad.data(), sizeof(int4) * lad.size(), hipMemcpyHostToDevice, s));
+    BatchMem mem;
+    mem.bases = (int *)(b + o_bases);
+    mem.queue = DevQueue{(int *)(b + o_queue), (int *)(b + o_queue) + 1, (int)ntask, 1}; // cleared by a kernel per chunk
+    mem.jobs = (DevJob *)(b + o_jobs);
+    mem.tasks = (DevTask *)(b + o_tasks);
+    mem.tail = tail.empty() ? nullptr : (DevTask *)(b + o_tail);
+    mem.ladders = (int4 *)(b + o_lad);
+    // no graph: instantiating one may allocate
+    if (int rc = batch_launches(ctx, t, opts, chunk, W, max_rep, (int)ntask, (int)lad.size(), mem, false, elapsed_ms))
+        return rc;
+    batch_done(js, shapes, opts, chunk);
     return GD_OK;
 }

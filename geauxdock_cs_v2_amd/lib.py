@@ -95,6 +95,8 @@ def _declare_device(lib):
     _sig(lib, "gd_set_stream", i32, [vp, i32, u64, C.POINTER(u32), i32])
     _sig(lib, "gd_job_shape", i32, [vp, i32, pi])
     _sig(lib, "gd_batch_plan", i32, [vp, pi])
+    _sig(lib, "gd_batch_run_noalloc", i32, [vp, pi, i32, C.POINTER(D.BatchOpts), C.POINTER(C.c_double)])
+    _sig(lib, "gd_batch_noalloc_capacity", C.c_size_t, [])
     _sig(lib, "gd_read_mclog", i32, [vp, i32, pi, pi])
     _sig(lib, "gd_server_cancel", i32, [vp, i32])
     _sig(lib, "gd_server_drain", i32, [vp])

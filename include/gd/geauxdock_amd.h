@@ -187,6 +187,22 @@ typedef struct gd_batch_opts {
 
 int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
                  double *elapsed_ms);
+/* gd_batch_run without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The launch
+ * tables (job descriptors, tasks, work queue, exchange ladders, step and
+ * round counters) live in a fixed device array of gd_batch_noalloc_capacity()
+ * bytes per GPU, held until the process ends; a batch that needs more fails
+ * with GD_ERR_ARG before anything runs.  No hipMalloc, hipFree, graph or
+ * autotuner trial: shapes are explicit, the autotuner's cached choice or the
+ * packing heuristic; use_graph is ignored and elapsed_ms is host time;
+ * keep_records needs each job's record window to hold a chunk already.  The
+ * array keeps the last call's contents and calls that overlap on one GPU
+ * race: serialise them. */
+int gd_batch_run_noalloc(gd_ctx *ctx, const int *job_ids, int n_jobs, const gd_batch_opts *opts,
+                         double *elapsed_ms);
+/* bytes of the device array gd_batch_run_noalloc's tables live in */
+size_t gd_batch_noalloc_capacity(void);
 /* Workgroup shape of the job's last batched launch (no counterpart in the
  * reference, whose block shape is a build constant, TperB): shape[0..3] =
  * waves per workgroup, slot waves, point slices, replicas per workgroup. */
