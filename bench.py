@@ -270,12 +270,22 @@ def run_serve(args, dev, perf, synth, Server):
                 dev.exchange(j, k)
         plain_dur.append(time.perf_counter() - t0)
         dev.free_job(j)
+        if (i + 1) % 25 == 0:
+            log(f"serve plain: {i + 1} of {len(cxs)} requests, {sum(plain_dur):.1f} s")
     plain_value = sum(poses) / sum(plain_dur)
     log(f"serve plain: {len(cxs)} requests run one after another, {sum(plain_dur):.2f} s, {plain_value:.4g} poses/s")
     rates = []
     rng = np.random.default_rng(7)
-    for lam in synth.SERVE["rates"]:
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(cxs)))
+    gaps = rng.exponential(1.0, len(cxs))  # Poisson arrivals of unit rate, scaled per load
+    capacity = None                           # requests/s the server sustains
+    # the workload's arrival rates, highest first: the highest is far above
+    # what one GPU serves, so its span gives the capacity; then loads below it
+    runs = [(lam, None) for lam in sorted(synth.SERVE["rates"], reverse=True)]
+    runs += [(None, load) for load in synth.SERVE["loads"]]
+    for lam, load in runs:
+        if lam is None:
+            lam = load * capacity
+        arrivals = np.cumsum(gaps) / lam
         srv = Server(dev, t, hbm_budget=args.serve_hbm_gib << 30, max_active=args.serve_max_active, chunk=100,
                      exchange=100)
         ids = []
@@ -285,16 +295,20 @@ def run_serve(args, dev, perf, synth, Server):
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(c, steps[i], 900 + i))
+            if (i + 1) % 50 == 0:
+                log(f"serve: {i + 1} of {len(cxs)} requests submitted, {time.perf_counter() - t0:.1f} s")
         srv.drain()
         res = [srv.poll(i)[0] for i in ids]
         srv.close()
         if any(r.status != 0 for r in res):
-            raise RuntimeError(f"serving at {lam}/s: failed requests {[r.status for r in res]}")
+            raise RuntimeError(f"serving at {lam:g}/s: failed requests {[r.status for r in res]}")
         lat = np.array([r.t_done_ms - r.t_submit_ms for r in res])
         queue = np.array([r.t_admit_ms - r.t_submit_ms for r in res])
         span = max(r.t_done_ms for r in res) - min(r.t_submit_ms for r in res)
         achieved = sum(poses) / (span * 1e-3)
         tflops = sum(flops) / (span * 1e-3) / 1e12
+        if capacity is None:
+            capacity = len(cxs) / (span * 1e-3)
         # the same arrivals served one after another by plain launches
         # (measured durations, first come first served)
         free_at, plain_lat = 0.0, []
@@ -302,24 +316,28 @@ def run_serve(args, dev, perf, synth, Server):
             start = max(a, free_at)
             free_at = start + dur
             plain_lat.append((free_at - a) * 1e3)
-        rates.append({"rate_per_s": lam, "requests": len(ids), "max_active": args.serve_max_active,
+        rates.append({"load": load, "rate_per_s": lam, "requests": len(ids),
+                      "max_active": args.serve_max_active,
                       "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                       "queue_p50_ms": float(np.percentile(queue, 50)), "queue_p99_ms": float(np.percentile(queue, 99)),
                       "queue_share": float(queue.sum() / lat.sum()),
                       "value": achieved, "tflops": tflops, "frac": frac(tflops, perf),
-                      "offered": lam * float(np.mean(poses)), "speedup_vs_plain": achieved / plain_value,
+                      "speedup_vs_plain": achieved / plain_value,
                       "plain_p50_ms": float(np.percentile(plain_lat, 50)),
                       "plain_p99_ms": float(np.percentile(plain_lat, 99))})
-        log(f"serve {lam:g}/s: p50 {rates[-1]['p50_ms']:.0f} ms, p99 {rates[-1]['p99_ms']:.0f} ms, queueing "
-            f"{100 * rates[-1]['queue_share']:.0f}% of latency, {achieved:.4g} poses/s ({tflops:.1f} TFLOP/s, "
-            f"offered {rates[-1]['offered']:.4g}); serial plain p50 {rates[-1]['plain_p50_ms']:.0f} ms, "
-            f"p99 {rates[-1]['plain_p99_ms']:.0f} ms")
+        name = f"{lam:.3g}/s" + ("" if load is None else f" (load {load:g})")
+        log(f"serve {name}: p50 {rates[-1]['p50_ms']:.0f} ms, p99 {rates[-1]['p99_ms']:.0f} ms, queueing "
+            f"{100 * rates[-1]['queue_share']:.0f}% of latency, {achieved:.4g} poses/s ({tflops:.1f} TFLOP/s); "
+            f"serial plain p50 {rates[-1]['plain_p50_ms']:.0f} ms, p99 {rates[-1]['plain_p99_ms']:.0f} ms")
     dev.free_target(t)
     return {"workload": f"serve: {len(cxs)} requests per rate, {min(l['n_atoms'] for l in info['ligands'])}-"
                         f"{max(l['n_atoms'] for l in info['ligands'])} atoms, 4-32 conf, 8 temperatures, "
                         f"{min(steps)}-{max(steps)} steps, against an {cxs[0].size['prt_npoint']}-point protein "
-                        f"x {cxs[0].size['n_prt']} conf, at most {args.serve_max_active} requests admitted at once",
-            "rates": rates, "plain": {"value": plain_value, "sample": "every step of each request, one request "
+                        f"x {cxs[0].size['n_prt']} conf, at most {args.serve_max_active} requests admitted at once; "
+                        f"Poisson arrivals at {', '.join(f'{r:g}' for r in synth.SERVE['rates'])}/s, then at "
+                        f"{', '.join(f'{l:g}' for l in synth.SERVE['loads'])} of the capacity measured at the "
+                        f"highest rate",
+            "capacity_per_s": capacity, "rates": rates, "plain": {"value": plain_value, "sample": "every step of each request, one request "
                                                                        "after another, plain launches"}}
 
 
@@ -413,8 +431,9 @@ def main():
         "serve": serve,
     }
     if serve:
-        out["p50_ms"] = {str(r["rate_per_s"]): r["p50_ms"] for r in serve["rates"]}
-        out["p99_ms"] = {str(r["rate_per_s"]): r["p99_ms"] for r in serve["rates"]}
+        key = lambda r: f"{r['rate_per_s']:.3g}/s" + ("" if r["load"] is None else f" (load {r['load']:g})")
+        out["p50_ms"] = {key(r): r["p50_ms"] for r in serve["rates"]}
+        out["p99_ms"] = {key(r): r["p99_ms"] for r in serve["rates"]}
     print(json.dumps(out))
 
 

@@ -25,6 +25,9 @@ ARCH = "gfx950"
 HOST_FLAGS = ["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", f"-I{INC}"]
 DEV_FLAGS = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", f"-I{INC}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # loop heads on 64-byte instruction-cache lines (the pair loops
+             # measured 0.2-0.3 % faster)
+             "-falign-loops=64",
              "-Wno-unused-command-line-argument"]
 
 

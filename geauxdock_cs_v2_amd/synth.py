@@ -329,7 +329,8 @@ SCREEN = dict(n_ligands=1024, median_atoms=32, log_sigma=0.5, min_atoms=12, max_
 # (uniform), 4-32 conformations, 8 temperatures, 500-5,000 steps
 # (log-uniform), Poisson arrivals at 50, 200 and 800 per second
 SERVE = dict(n_points=8000, n_prt_conf=4, prt_sigma=0.5, prt_radius=35.0, atoms=(10, 120), confs=(4, 32),
-             n_tmp=8, t_lo=0.3, t_hi=2.0, steps=(500, 5000), rates=(50.0, 200.0, 800.0), lig_sigma=0.4)
+             n_tmp=8, t_lo=0.3, t_hi=2.0, steps=(500, 5000), rates=(50.0, 200.0, 800.0),
+             loads=(0.8, 0.5), lig_sigma=0.4)
 
 
 def _ligand_files(d, lig_id, rng_seed, n_atoms, n_conf, lig_sigma, protein):

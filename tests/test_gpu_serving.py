@@ -159,3 +159,51 @@ def test_exchange_rounds_fall_due_whatever_the_chunk(setup):
     labels = lambda reps: [r.idx_tmp for r in reps]
     assert labels(c[3]) == labels(start)
     assert labels(a[3]) != labels(start)  # four rounds at 50, 100, 150, 200 moved some
+
+
+def test_serving_workload_size_requests_match_fp64(lib, tmp_path_factory):
+    """Two requests of the serving workload's largest kind served together:
+    an 8,000-point protein x 4 conformations (35 A ball), ligands of 105
+    atoms x 8 conformations and 119 atoms x 32 conformations, 8 temperatures
+    (256 and 1,024 replicas), 300 and 200 steps with exchanges every 100.
+    The final energies of 64 replicas of each (every 4th, resp. 16th) are
+    compared with FP64 at their final moves, pocket-specific step ties
+    resolved as in the parity tests."""
+    from test_gpu_parity import psp_step_ties
+    d = str(tmp_path_factory.mktemp("serve_big"))
+    P = synth.SERVE
+    prot = synth._protein_only(d, "big", 77, P["n_points"], P["n_prt_conf"], P["prt_sigma"], P["prt_radius"])
+    specs = [("bigA", 105, 8, 300), ("bigB", 119, 32, 200)]
+    cxs = []
+    for k, (name, na, nc, _) in enumerate(specs):
+        f = synth._ligand_files(d, name, 300 + k, na, nc, P["lig_sigma"], prot)
+        raw = L.load_job(f["pdb"], f["sdf"], f["ff"], name, n_tmp=P["n_tmp"], temp_high=P["t_hi"],
+                         temp_low=P["t_lo"])
+        cxs.append(raw.stage())
+    assert [c.size["n_rep"] for c in cxs] == [256, 1024]
+    assert cxs[0].size["prt_npoint"] >= 8000 - 100 and cxs[0].size["n_prt"] == 4
+    h = C.c_void_p()
+    L.check(lib.gd_create(0, C.byref(h)))
+    t = C.c_int()
+    L.check(lib.gd_upload_target(h, C.byref(cxs[0].view), C.byref(t)))
+    try:
+        say(f"case: serving two requests of {[c.size['lig_natom'] for c in cxs]} atoms against "
+            f"{cxs[0].size['prt_npoint']} points x 4")
+        out = serve(lib, h, t.value, [(c, s, 31 + k) for k, (c, (_, _, _, s)) in enumerate(zip(cxs, specs))],
+                    chunk=100, exchange=100)
+    finally:
+        lib.gd_free_target(h, t.value)
+        lib.gd_destroy(h)
+    for c, (st, _, res, reps), (name, _, _, steps) in zip(cxs, out, specs):
+        assert st == 0
+        n = c.size["n_rep"]
+        idx = np.arange(0, n, n // 64, dtype=np.int32)
+        mm = np.array([list(reps[i].movematrix)[:6] for i in idx], np.float32)
+        eg = np.array([list(reps[i].energy)[:10] for i in idx])
+        e64 = O.energy(c.view, idx, mm, "f64")
+        bad, left = psp_step_ties(c, idx, mm, eg, e64)
+        err = rel_err(eg, e64)
+        say(f"{name}: {len(idx)} replicas, max rel err vs FP64 {err.max():.2e} "
+            f"({len(bad)} pocket-specific step ties, {len(left)} unexplained)")
+        assert not left
+        assert all(0 <= reps[i].step < steps for i in idx)
