@@ -470,6 +470,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     std::vector<float2> pa((size_t)kCenters * np);
     std::vector<float> pw((size_t)kCenters * np);
     std::vector<float4> pt((size_t)kCenters * nconf * np);
+    std::vector<float> pq((size_t)kCenters * nconf * np);
     std::vector<int> ladder((size_t)kCenters * nseg * kFarRungs), ladder_in((size_t)kCenters * nseg * kFarRungs);
     std::vector<double> dmin(np), dmax(np);
     std::vector<int> ordk(np);
@@ -502,12 +503,17 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
             pa[(size_t)kc * np + i] = v;
             pw[(size_t)kc * np + i] = v.x;
         }
+        // q = p - C_k from the frame coordinates the kernel's atoms use (the
+        // float local values), in double: q' = -2 q rounds once, |q|^2 once
         for (int c = 0; c < nconf; ++c)
             for (int i = 0; i < np; ++i) {
                 const size_t s = (size_t)c * np + ordk[i];
-                pt[((size_t)kc * nconf + c) * np + i] =
-                    make_float4(local(cx->prt_x[s], origin[0]), local(cx->prt_y[s], origin[1]),
-                                local(cx->prt_z[s], origin[2]), cx->prt_ele[ordk[i]]);
+                const double qx = (double)local(cx->prt_x[s], origin[0]) - cc,
+                             qy = (double)local(cx->prt_y[s], origin[1]) - cc,
+                             qz = (double)local(cx->prt_z[s], origin[2]) - cc;
+                const size_t o = ((size_t)kc * nconf + c) * np + i;
+                pt[o] = make_float4((float)(-2.0 * qx), (float)(-2.0 * qy), (float)(-2.0 * qz), cx->prt_ele[ordk[i]]);
+                pq[o] = (float)(qx * qx + qy * qy + qz * qz);
             }
         for (size_t g = 0; g < nseg; ++g)
             for (int k = 0; k < kFarRungs; ++k) {
@@ -536,6 +542,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         }
     } drop{ctx, id};
     GD_HIP(upload(t.pt, pt.data(), sizeof(float4) * pt.size(), ctx->stream));
+    GD_HIP(upload(t.pq, pq.data(), sizeof(float) * pq.size(), ctx->stream));
     GD_HIP(upload(t.pa, pa.data(), sizeof(float2) * pa.size(), ctx->stream));
     GD_HIP(upload(t.pw, pw.data(), sizeof(float) * pw.size(), ctx->stream));
     GD_HIP(upload(t.pp, pp.data(), sizeof(float) * pp.size(), ctx->stream));
@@ -553,6 +560,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
     t.dt.inv_center_dr = 1.0f / center_dr;
     t.dt.prot_r = (float)t_prot_r + 1e-3f; // rounded up: a rung at or beyond it holds every point
     t.dt.pt = t.pt.as<float4>();
+    t.dt.pq = t.pq.as<float>();
     t.dt.pa = t.pa.as<float2>();
     t.dt.pw = t.pw.as<float>();
     t.dt.pp = t.pp.as<float>();
@@ -594,7 +602,7 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         t.seg_host.push_back(g.y);
         t.seg_host.push_back(g.z);
     }
-    t.bytes = t.pt.bytes + t.pa.bytes + t.pw.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes + t.ladder_in.bytes;
+    t.bytes = t.pt.bytes + t.pq.bytes + t.pa.bytes + t.pw.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes + t.ladder_in.bytes;
     t.dev_index.assign(np, 0);
     for (int i = 0; i < np; ++i) t.dev_index[ord[i]] = i;
     t.prt_t.assign(cx->prt_t, cx->prt_t + np);

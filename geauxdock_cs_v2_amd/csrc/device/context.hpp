@@ -60,7 +60,7 @@ struct DevBuf {
 };
 
 struct TargetRes {
-    DevBuf pt, pa, pw, pp, seg, ladder, ladder_in;
+    DevBuf pt, pq, pa, pw, pp, seg, ladder, ladder_in;
     DevTarget dt{};
     DevScalars sc{};
     int n_prt = 0, npoint = 0;

@@ -2,13 +2,15 @@
 // uploads and launches them.
 //
 // HBM layout (one resident target, any number of jobs):
-//   target  pt   float4[kCenters][nconf][npad]  {x, y, z, ele} of every
-//                                     effective point in a frame centred on the
-//                                     pocket centre, re-sorted by (point type,
-//                                     hydrophobic flag) so a run of points (a
-//                                     segment) shares one set of type-pair
-//                                     parameters and terms; copy k orders each
-//                                     segment by distance from ladder centre k
+//   target  pt   float4[kCenters][nconf][npad]  {-2 (p - C_k), ele} of every
+//                                     effective point p (frame centred on the
+//                                     pocket centre; C_k the ladder centre of
+//                                     copy k, C_0 = 0), re-sorted by (point
+//                                     type, hydrophobic flag) so a run of
+//                                     points (a segment) shares one set of
+//                                     type-pair parameters and terms; copy k
+//                                     orders each segment by distance from C_k
+//           pq   float[kCenters][nconf][npad] |p - C_k|^2
 //           pa   float2[kCenters][npad] {hpc weight = (cdc ? hpp : 0) / 2, point type}
 //           pw   float[kCenters][npad] the hpc weights of pa alone
 //           seg  int4[nseg]           {point type, begin, end, hydrophobic flag}
@@ -85,7 +87,11 @@ struct DevTarget {
     // pt, pa and the ladders come in kCenters copies: copy k orders each
     // segment's points by their distance from ladder centre k (copy 0: the
     // pocket centre); the segments themselves are the same in every copy
-    const float4 *pt;  // [kCenters][nconf][npad] {x, y, z, ele}, pocket-centred frame
+    // copy k holds q' = -2 (p - C_k): the direct difference x - p is
+    // fma(1/2, q', x - C_k) exactly, and |x - p|^2 = |u|^2 + |q|^2 + u.q'
+    // for u = x - C_k (see pq)
+    const float4 *pt;  // [kCenters][nconf][npad] {q'x, q'y, q'z, ele}
+    const float *pq;   // [kCenters][nconf][npad] |p - C_k|^2
     const float2 *pa;  // [kCenters][npad] {hydrophobic weight / 2, point type (int bits)}
     const float *pw;   // [kCenters][npad] hydrophobic weight / 2 alone, contiguous for wide scalar loads
     const float *pp;   // [31][24][PP_N] type-pair parameters

@@ -74,11 +74,12 @@ __device__ __forceinline__ void acc_add(float &s, float &c, float v)
 }
 
 struct Slot {
-    float x, y, z; // transformed atom
+    float x, y, z; // transformed atom, relative to the ladder centre of the point table in use
     int tl;        // ligand type
 };
 
 constexpr int kBlock = 64; // points per plain FP32 partial sum
+constexpr float kExpandR = 16.0f; // A: largest distance of a wave's atoms from its ladder centre for expanded distances
 constexpr int kVals = 9;   // per-slot terms: vdw, ele hi, pmf, psp, hdb, hpc, kde, mcs, ele lo
 constexpr int kRed = 9;    // per-slot partials of a slice: vdw, ele, pmf, psp, hdb, hpc, ele compensation, kde, mcs
 static_assert(kRed <= kVals, "the cross-slice partials share the per-slot terms' LDS region");
@@ -196,7 +197,7 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_,
         for (int j = 0; j < 4; ++j) {
             P[j] = make_float4(g4[4 * j], g4[4 * j + 1], g4[4 * j + 2], g4[4 * j + 3]);
             w[j] = HPC ? w4[j] : 0.0f;
-            const float dx = me.x - P[j].x, dy = me.y - P[j].y, dz = me.z - P[j].z;
+            const float dx = fmaf(0.5f, P[j].x, me.x), dy = fmaf(0.5f, P[j].y, me.y), dz = fmaf(0.5f, P[j].z, me.z);
             d2[j] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             rinv[j] = __builtin_amdgcn_rsqf(d2[j]);
         }
@@ -218,7 +219,7 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_,
     }
     for (; p < pe; ++p) {
         const float4 P4 = ld_point(ptc, p);
-        const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+        const float dx = fmaf(0.5f, P4.x, me.x), dy = fmaf(0.5f, P4.y, me.y), dz = fmaf(0.5f, P4.z, me.z);
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float rinv = __builtin_amdgcn_rsqf(d2);
         pair_terms<HPC, HDB>(d2, rinv, HPC ? pw[p] : 0.0f, q, sc, s);
@@ -229,18 +230,34 @@ __device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_,
 // Far field: points at least far_d from every atom of the wave (see
 // DevTarget::far_d).  Only LJ and 1/d electrostatics vary there; the contact
 // term is pmf1 per point, the hydrophobic, core and H-bond terms are 0.
-__device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int pe, const Slot &me,
-                                               const PairParams &q, const DevScalars &sc, Sums &s)
+// Squared distance of the atom me from point P (q' = P.xyz, see
+// DevTarget::pt).  EXPAND: |u|^2 + |q|^2 + u.q', one add and three FMAs
+// instead of three subtractions and three multiply-adds; its rounding error
+// is about 2^-22 (|u| + |q|)^2, so it is used only where d is large against
+// |u| + |q| (far and very far pairs of waves near their ladder centre, and
+// the lean path).
+template <bool EXPAND>
+__device__ __forceinline__ float dist2(const float4 &P, float qq, const Slot &me, float uu)
+{
+    if (EXPAND) return fmaf(me.x, P.x, fmaf(me.y, P.y, fmaf(me.z, P.z, uu + qq)));
+    const float dx = fmaf(0.5f, P.x, me.x), dy = fmaf(0.5f, P.y, me.y), dz = fmaf(0.5f, P.z, me.z);
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+template <bool EXPAND>
+__device__ __forceinline__ void pair_block_far(const float4 *ptc_, const float *pq_, int pb, int pe, const Slot &me,
+                                               float uu, const PairParams &q, const DevScalars &sc, Sums &s)
 {
     const cfloat *ptc = (const cfloat *)ptc_;
+    const cfloat *pq = (const cfloat *)pq_;
     int p = pb;
     for (; p + 4 <= pe; p += 4) {
         float ge = 0.0f;
+        const cfloat *q4 = pq + p;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float4 P4 = ld_point(ptc, p + j);
-            const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
-            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float d2 = dist2<EXPAND>(P4, EXPAND ? q4[j] : 0.0f, me, uu);
             const float rinv = __builtin_amdgcn_rsqf(d2);
             lj_term(d2, d2 * rinv, q, sc, s);
             ge = fmaf(P4.w, rinv, ge);
@@ -249,8 +266,7 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int p
     }
     for (; p < pe; ++p) {
         const float4 P4 = ld_point(ptc, p);
-        const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float d2 = dist2<EXPAND>(P4, EXPAND ? pq[p] : 0.0f, me, uu);
         const float rinv = __builtin_amdgcn_rsqf(d2);
         lj_term(d2, d2 * rinv, q, sc, s);
         kahan_add(s.e, s.ec, P4.w * rinv);
@@ -267,11 +283,8 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, int pb, int p
 // LJ of one very far pair into s.v (TIER 2: its attractive part into v2);
 // returns 1/d for the electrostatics
 template <int TIER>
-__device__ __forceinline__ float vfar_lj(const float4 &P4, const Slot &me, const PairParams &q, const DevScalars &sc,
-                                         Sums &s, float &v2)
+__device__ __forceinline__ float vfar_lj(float d2, const PairParams &q, const DevScalars &sc, Sums &s, float &v2)
 {
-    const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
-    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = __builtin_amdgcn_rsqf(d2);
     const float r2 = rinv * rinv;
     const float r3 = r2 * rinv;
@@ -295,26 +308,30 @@ __device__ __forceinline__ float vfar_lj(const float4 &P4, const Slot &me, const
     return rinv;
 }
 
-template <int TIER>
-__device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, int pb, int pe, const Slot &me,
-                                                const PairParams &q, const DevScalars &sc, Sums &s)
+template <int TIER, bool EXPAND>
+__device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, const float *pq_, int pb, int pe, const Slot &me,
+                                                float uu, const PairParams &q, const DevScalars &sc, Sums &s)
 {
     const cfloat *ptc = (const cfloat *)ptc_;
+    const cfloat *pq = (const cfloat *)pq_;
     int p = pb;
     float v2 = 0.0f;
     for (; p + 4 <= pe; p += 4) {
         float ge = 0.0f;
+        const cfloat *q4 = pq + p;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float4 P4 = ld_point(ptc, p + j);
-            ge = fmaf(P4.w, vfar_lj<TIER>(P4, me, q, sc, s, v2), ge);
+            const float d2 = dist2<EXPAND>(P4, EXPAND ? q4[j] : 0.0f, me, uu);
+            ge = fmaf(P4.w, vfar_lj<TIER>(d2, q, sc, s, v2), ge);
         }
         kahan_add(s.e, s.ec, ge);
     }
     // the last points of the block, with the same arithmetic
     for (; p < pe; ++p) {
         const float4 P4 = ld_point(ptc, p);
-        kahan_add(s.e, s.ec, P4.w * vfar_lj<TIER>(P4, me, q, sc, s, v2));
+        const float d2 = dist2<EXPAND>(P4, EXPAND ? pq[p] : 0.0f, me, uu);
+        kahan_add(s.e, s.ec, P4.w * vfar_lj<TIER>(d2, q, sc, s, v2));
     }
     if (TIER == 2) s.v -= v2;
     s.p = fmaf(q.pmf1, (float)(pe - pb), s.p);
@@ -572,10 +589,17 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         const int kc = __builtin_amdgcn_readfirstlane(kpos > (float)(kCenters - 1) ? 0 : (int)fmaxf(kpos, 0.0f));
         const float cc = (float)kc * tg.center_dr * 0.57735026918962576f;
         const float4 *__restrict__ ptc = tg.pt + ((size_t)kc * tg.nconf + prt) * tg.npad;
+        const float *__restrict__ pqc = tg.pq + ((size_t)kc * tg.nconf + prt) * tg.npad;
         const float *__restrict__ pak = tg.pw + (size_t)kc * tg.npad;
         const float ux = me.x - cc, uy = me.y - cc, uz = me.z - cc;
         const float r2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+        // the atom relative to the ladder centre, as the point table holds the points
+        const Slot mu{ux, uy, uz, me.tl};
         const float rho_out = sqrtf(wave_max(r2)), rho_in = sqrtf(wave_min(r2));
+        // far and very far pairs (d >= far_d) take the expanded distance when
+        // the wave lies within kExpandR of its ladder centre: (|u| + |q|) / d
+        // then stays below (2 kExpandR + far_d) / far_d
+        const bool expand = rho_out <= kExpandR;
         const int kf = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho_out + tg.far_d - kFarR0) * (1.0f / kFarDr))));
         // the same rungs for the mid field (mid_d <= far_d, so km <= kf and kmi >= ki)
         const int km = __builtin_amdgcn_readfirstlane(max(0, (int)ceilf((rho_out + tg.mid_d - kFarR0) * (1.0f / kFarDr))));
@@ -620,12 +644,14 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 for (int pb = b; pb < e; pb += kBlock) {
                     const int pe = min(pb + kBlock, e);
                     Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    // lean: |q| <= prot_r and d >= |u| - prot_r >= vfar_d, so
+                    // (|u| + |q|) / d <= (2 prot_r + vfar_d) / vfar_d
                     if (tier == 2)
-                        pair_block_vfar<2>(ptc, pb, pe, me, q, sc, bs);
+                        pair_block_vfar<2, true>(ptc, pqc, pb, pe, mu, r2, q, sc, bs);
                     else if (tier == 1)
-                        pair_block_vfar<1>(ptc, pb, pe, me, q, sc, bs);
+                        pair_block_vfar<1, true>(ptc, pqc, pb, pe, mu, r2, q, sc, bs);
                     else
-                        pair_block_vfar<0>(ptc, pb, pe, me, q, sc, bs);
+                        pair_block_vfar<0, true>(ptc, pqc, pb, pe, mu, r2, q, sc, bs);
                     acc_add<KAHAN>(evdw, cvdw, bs.v);
                     acc_add<KAHAN>(eele, cele, bs.e);
                     acc_add<KAHAN>(eele, cele, -bs.ec);
@@ -667,14 +693,14 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                 Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (sg.w) { // segment flag: hydrophobic-contact points
                     if (hdb)
-                        pair_block<true, true>(ptc, pak, pb, pe, me, q, sc, bs);
+                        pair_block<true, true>(ptc, pak, pb, pe, mu, q, sc, bs);
                     else
-                        pair_block<true, false>(ptc, pak, pb, pe, me, q, sc, bs);
+                        pair_block<true, false>(ptc, pak, pb, pe, mu, q, sc, bs);
                 } else {
                     if (hdb)
-                        pair_block<false, true>(ptc, pak, pb, pe, me, q, sc, bs);
+                        pair_block<false, true>(ptc, pak, pb, pe, mu, q, sc, bs);
                     else
-                        pair_block<false, false>(ptc, pak, pb, pe, me, q, sc, bs);
+                        pair_block<false, false>(ptc, pak, pb, pe, mu, q, sc, bs);
                 }
                 acc_add<KAHAN>(evdw, cvdw, bs.v);
                 acc_add<KAHAN>(eele, cele, bs.e);
@@ -692,11 +718,18 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
                     const int pe = min(pb + kBlock, fe);
                     Sums bs{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                     if (part < 2)
-                        pair_block<false, false, false>(ptc, pak, pb, pe, me, q, sc, bs);
-                    else if (part < 4)
-                        pair_block_far(ptc, pb, pe, me, q, sc, bs);
-                    else
-                        pair_block_vfar<0>(ptc, pb, pe, me, q, sc, bs);
+                        pair_block<false, false, false>(ptc, pak, pb, pe, mu, q, sc, bs);
+                    else if (part < 4) {
+                        if (expand)
+                            pair_block_far<true>(ptc, pqc, pb, pe, mu, r2, q, sc, bs);
+                        else
+                            pair_block_far<false>(ptc, pqc, pb, pe, mu, r2, q, sc, bs);
+                    } else {
+                        if (expand)
+                            pair_block_vfar<0, true>(ptc, pqc, pb, pe, mu, r2, q, sc, bs);
+                        else
+                            pair_block_vfar<0, false>(ptc, pqc, pb, pe, mu, r2, q, sc, bs);
+                    }
                     acc_add<KAHAN>(evdw, cvdw, bs.v);
                     acc_add<KAHAN>(eele, cele, bs.e);
                     acc_add<KAHAN>(eele, cele, -bs.ec);
@@ -712,9 +745,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
             for (int k = 0; k < J.npsp; ++k) {
                 const int p = J.psp_pt[k];
                 if (p < lo || p >= hi) continue; // the slice that owns the point adds it
-                const float4 P4 = ptc0[p];
+                const float4 P4 = ptc0[p]; // centre 0: q' = -2 p
                 const int tp = __float_as_int(tg.pa[p].y);
-                const float dx = me.x - P4.x, dy = me.y - P4.y, dz = me.z - P4.z;
+                const float dx = fmaf(0.5f, P4.x, me.x), dy = fmaf(0.5f, P4.y, me.y), dz = fmaf(0.5f, P4.z, me.z);
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 const float d = d2 * __builtin_amdgcn_rsqf(d2);
                 const float pmf0 = tg.pp[((size_t)tp * GD_MAXTP2 + me.tl) * PP_N + PP_PMF0];
