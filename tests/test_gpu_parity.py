@@ -481,6 +481,38 @@ def test_far_field_energies_large_protein(lib, ctx):
     assert err.max() <= REL_TOL, err
 
 
+@pytest.mark.parametrize("shift", [2.0, 4.0, 6.0, 9.0, 11.0, 14.0])
+def test_expanded_distances_near_their_bounds(lib, ctx, shift):
+    """Far and very far pairs take |u|^2 + |q|^2 + u.q' for waves within
+    16 A of their ladder centre (u the atom, q the point, relative to it),
+    whose rounding grows with |u| + |q|.  Ligands of the serving workload
+    (up to 119 atoms, about 6 A across) translated by 2-14 A in random
+    directions put their waves' farthest atom on both sides of that bound;
+    every term stays within tolerance of FP64."""
+    from geauxdock_cs_v2_amd import synth
+    info = synth.many_files("serve", n=4)
+    cx = max(synth.load_many(info, threads=4), key=lambda c: c.size["lig_natom"])
+    n_rep = cx.size["n_rep"]
+    rng = np.random.default_rng(int(shift * 10))
+    n = min(96, n_rep)
+    idx = rng.choice(n_rep, n, replace=False).astype(np.int32)
+    v = rng.normal(size=(n, 3))
+    v = v / np.linalg.norm(v, axis=1, keepdims=True) * shift
+    move = np.concatenate([v, rng.uniform(-math.pi, math.pi, (n, 3))], 1).astype(np.float32)
+    say(f"case: expanded distances, shift {shift} A, {cx.size['lig_natom']} atoms, "
+        f"{cx.size['prt_npoint']} points, n={n}")
+    t, j = upload(lib, ctx, cx)
+    eg = gpu_energy(lib, ctx, j, idx, move)
+    free(lib, ctx, t, j)
+    e64 = O.energy(cx.view, idx, move, "f64")
+    bad, left = psp_step_ties(cx, idx, move, eg, e64)
+    err = rel_err(eg, e64).max(0)
+    say(f"shift {shift} A max rel err vs FP64:", np.array2string(err, precision=2, max_line_width=200),
+        f"({len(bad)} pocket-specific step ties)")
+    assert np.all(np.isfinite(eg))
+    assert not left
+
+
 def test_explicit_stream_words(lib, ctx):
     """gd_set_stream with explicit words: a job fed the Philox words of seed B
     as a buffer makes exactly the moves and decisions of a job keyed by B,
