@@ -180,7 +180,7 @@ int run_tasks(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const std
               int waves, int max_rep, bool compensated, bool records)
 {
     GD_HIP(upload(ctx->task_buf, tasks.data(), sizeof(DevTask) * tasks.size(), ctx->stream));
-    GD_HIP(launch_mc(t.dt, t.sc, jobs_dev, ctx->task_buf.as<DevTask>(), (int)tasks.size(), waves, max_rep,
+    GD_HIP(launch_mc(t.params.as<DevParams>(), jobs_dev, ctx->task_buf.as<DevTask>(), (int)tasks.size(), waves, max_rep,
                      compensated, records, ctx->stream, nullptr));
     return GD_OK;
 }
@@ -602,7 +602,11 @@ int gd_upload_target(gd_ctx *ctx, const gd_Complex *cx, int *target_id)
         t.seg_host.push_back(g.y);
         t.seg_host.push_back(g.z);
     }
-    t.bytes = t.pt.bytes + t.pq.bytes + t.pa.bytes + t.pw.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes + t.ladder_in.bytes;
+    t.bytes = t.pt.bytes + t.pq.bytes + t.pa.bytes + t.pw.bytes + t.pp.bytes + t.seg.bytes + t.ladder.bytes + t.ladder_in.bytes +
+              sizeof(DevParams);
+    // the host copy stays alive until the stream synchronisation below
+    const DevParams prm_host{t.dt, t.sc};
+    GD_HIP(upload(t.params, &prm_host, sizeof(prm_host), ctx->stream));
     t.dev_index.assign(np, 0);
     for (int i = 0; i < np; ++i) t.dev_index[ord[i]] = i;
     t.prt_t.assign(cx->prt_t, cx->prt_t + np);
@@ -913,7 +917,7 @@ int gd_energy(gd_ctx *ctx, int job_id, int n, const int *rep_idx, const float *m
     GD_HIP(upload(ctx->pose_rep, rep_idx, sizeof(int) * n, s));
     GD_HIP(upload(ctx->pose_move, move, sizeof(float) * 6 * n, s));
     if (ctx->pose_out.bytes < sizeof(float) * GD_MAXWEI * n) GD_HIP(ctx->pose_out.alloc(sizeof(float) * GD_MAXWEI * n));
-    GD_HIP(launch_energy(t.dt, t.sc, j->self.as<DevJob>(), ctx->task_buf.as<DevTask>(), n, sh.waves, true,
+    GD_HIP(launch_energy(t.params.as<DevParams>(), j->self.as<DevJob>(), ctx->task_buf.as<DevTask>(), n, sh.waves, true,
                          ctx->pose_rep.as<int>(), ctx->pose_move.as<float>(), ctx->pose_out.as<float>(), s));
     GD_HIP(hipMemcpyAsync(e_out, ctx->pose_out.p, sizeof(float) * GD_MAXWEI * n, hipMemcpyDeviceToHost, s));
     GD_HIP(hipStreamSynchronize(s));

@@ -213,7 +213,7 @@ int enqueue_mc_on(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const
     ctx->last_plan[1] = std::min(slots, ntask * q.nslice);
     ctx->last_plan[2] = q.nslice;
     GD_HIP(launch_queue_reset(q, s));
-    GD_HIP(launch_mc_queue(t.dt, t.sc, jobs_dev, tasks_dev, q, slots, waves, max_rep, comp, records, s, step_base));
+    GD_HIP(launch_mc_queue(t.params.as<DevParams>(), jobs_dev, tasks_dev, q, slots, waves, max_rep, comp, records, s, step_base));
     return GD_OK;
 }
 

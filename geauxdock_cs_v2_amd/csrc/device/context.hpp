@@ -61,6 +61,7 @@ struct DevBuf {
 
 struct TargetRes {
     DevBuf pt, pq, pa, pw, pp, seg, ladder, ladder_in;
+    DevBuf params; // DevParams {dt, sc} in device memory, read by the kernels
     DevTarget dt{};
     DevScalars sc{};
     int n_prt = 0, npoint = 0;

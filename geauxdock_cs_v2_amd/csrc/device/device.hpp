@@ -111,6 +111,15 @@ struct DevTarget {
     int npoint, npad, nseg, nconf;
 };
 
+// The target's tables and scalars as the Monte Carlo kernel reads them: one
+// copy in device memory per uploaded target.  The kernel reads its fields
+// with scalar loads where it needs them (instead of receiving them as
+// by-value arguments the compiler keeps in registers for the whole launch).
+struct DevParams {
+    DevTarget tg;
+    DevScalars sc;
+};
+
 struct DevJob {
     const float4 *lig3;
     const float4 *lig2;
@@ -178,17 +187,16 @@ size_t mc_lds_bytes(int waves, int nrep);
 // `waves` waves each (every task's slot_waves * slices == waves).
 // step_base (device int, may be null) is added to every task's s1, so a
 // captured graph can advance the step counter without new kernel arguments.
-hipError_t launch_mc(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
-                     int ntask, int waves, int max_rep, bool compensated, bool keep_records,
-                     hipStream_t stream, const int *step_base);
+hipError_t launch_mc(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, int ntask, int waves,
+                     int max_rep, bool compensated, bool keep_records, hipStream_t stream, const int *step_base);
 
 // Persistent form: `grid` workgroups walk the queue q (q.ntask tasks in
 // q.nslice step slices each).  The queue's counter and done[] must be zero
 // at launch (launch_queue_reset clears them on the stream, with a kernel, so
 // the sequence can be captured in a graph).
-hipError_t launch_mc_queue(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
-                           const DevQueue &q, int grid, int waves, int max_rep, bool compensated,
-                           bool keep_records, hipStream_t stream, const int *step_base);
+hipError_t launch_mc_queue(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q,
+                           int grid, int waves, int max_rep, bool compensated, bool keep_records,
+                           hipStream_t stream, const int *step_base);
 hipError_t launch_queue_reset(const DevQueue &q, hipStream_t stream);
 
 // Workgroups of the Monte Carlo kernel resident on one CU for this shape
@@ -196,9 +204,9 @@ hipError_t launch_queue_reset(const DevQueue &q, hipStream_t stream);
 int mc_blocks_per_cu(int waves, int max_rep);
 
 // Energy mode: poses i = task.rep0 (nrep == 1), replica pose_rep[i], move pose_move[6i..]
-hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
-                         int ntask, int waves, bool compensated, const int *pose_rep, const float *pose_move,
-                         float *pose_out, hipStream_t stream);
+hipError_t launch_energy(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, int ntask, int waves,
+                         bool compensated, const int *pose_rep, const float *pose_move, float *pose_out,
+                         hipStream_t stream);
 
 // One exchange round over ladders ladder0 .. ladder0 + n_ladder - 1 of a job
 // (ladder p * n_lig + l: ligand conformation l, protein conformation p).

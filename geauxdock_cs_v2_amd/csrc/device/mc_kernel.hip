@@ -51,6 +51,38 @@ __device__ __forceinline__ float uniform31(uint32_t w) { return (float)(w >> 1) 
 
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
+// sin and cos of a cumulative Euler angle.  Reduction by Cody and Waite
+// (Software Manual for the Elementary Functions, 1980): k = rint(x / (pi/2)),
+// r = x - k pi/2 with pi/2 split into three floats P1 + P2 + P3 (P1 the float
+// nearest pi/2, P2 and P3 the float-rounded remainders).  fma(-k, P1, x) is
+// exact for |k| < 2^23 (k P1 and x share the grid 2^-23 min(1, ulp x) and the
+// difference is below 1 in magnitude), the P2 and P3 steps round once each,
+// so r carries about one ulp for the angles a run reaches (|x| up to 10^6
+// rad; the moves add at most rot_scale per step).  On |r| <= pi/4: Taylor
+// polynomials of sin through r^9 (next term below 2.5e-9) and of cos through
+// r^10 (below 1.2e-10); quadrant from k mod 4.  No tables and no wide-argument
+// branch, so no constants stay resident in registers across the step.
+__device__ __forceinline__ void sincos_cw(float x, float &s, float &c)
+{
+    const float kf = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-kf, 1.57079637050628662f, x);
+    r = fmaf(-kf, -4.37113882867379e-08f, r);
+    r = fmaf(-kf, -1.71512451000588e-15f, r);
+    const int q = (int)kf;
+    const float r2 = r * r;
+    const float ps = fmaf(r2, fmaf(r2, fmaf(r2, 1.0f / 362880.0f, -1.0f / 5040.0f), 1.0f / 120.0f), -1.0f / 6.0f);
+    const float sn = fmaf(r * r2, ps, r);
+    const float pc = fmaf(r2, fmaf(r2, fmaf(r2, fmaf(r2, -1.0f / 3628800.0f, 1.0f / 40320.0f), -1.0f / 720.0f),
+                                   1.0f / 24.0f),
+                          -0.5f);
+    const float cs = fmaf(r2, pc, 1.0f);
+    // x = k pi/2 + r: quadrant k mod 4 rotates (sin r, cos r)
+    const bool odd = q & 1, neg_s = q & 2, neg_c = (q + 1) & 2;
+    const float ss = odd ? cs : sn, cc = odd ? sn : cs;
+    s = neg_s ? -ss : ss;
+    c = neg_c ? -cc : cc;
+}
+
 // scale * u + m rounded twice, as the reference computes the proposal
 // (kernel_cpp_montecarlo_soa.cpp:234); contraction into an FMA would change
 // the last bit of the cumulative move
@@ -93,6 +125,12 @@ struct PairParams {
     float p1a, p2a, pmf1, pmfb, pmfc, hk2, hk1, hk0, hdb1;
 };
 
+// The scalars the pair loops use (DevScalars), read once per step.
+struct PairConst {
+    float lj_a, lj_b, pmf_a;
+    float el_e0, el_a, el_b, el_dc2;
+};
+
 struct Sums {
     float v, e, p, h, q; // vdw (lj0-scaled), ele (el1-scaled, without the atom charge), pmf, hdb, hydrophobic
     float ec;            // compensation of e: the sum is e - ec
@@ -113,6 +151,19 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
 typedef const __attribute__((address_space(4))) float cfloat;
 typedef const __attribute__((address_space(4))) int cint;
 
+// A descriptor (DevParams, DevJob) read through the constant address space
+// from a point the compiler cannot move: the empty asm makes the pointer
+// opaque, so each field is fetched with a scalar load where the code uses it
+// instead of being hoisted to the kernel's start and held in SGPRs across
+// the whole launch (which spilled them into VGPR lanes).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T &uref(const T *p)
+{
+    auto q = (const __attribute__((address_space(4))) T *)p;
+    asm volatile("" : "+s"(q));
+    return *q;
+}
+
 __device__ __forceinline__ float4 ld_point(const cfloat *pt, int p)
 {
     return make_float4(pt[4 * p], pt[4 * p + 1], pt[4 * p + 2], pt[4 * p + 3]);
@@ -122,7 +173,7 @@ __device__ __forceinline__ float4 ld_point(const cfloat *pt, int p)
 // el1 (el0 + a1 s^2 + b1 s^3), s = el1 d, inside s < 1 (the core meets 1/s
 // with equal value and slope at s = 1).  Every pair adds ele / d; a pair
 // inside the core then adds ele (core - 1/d).
-__device__ __forceinline__ float ele_core_delta(float d2, float rinv, float ele, const DevScalars &sc)
+__device__ __forceinline__ float ele_core_delta(float d2, float rinv, float ele, const PairConst &sc)
 {
     const float d = d2 * rinv;
     const float core = fmaf(d2, fmaf(sc.el_b, d, sc.el_a), sc.el_e0);
@@ -135,7 +186,7 @@ __device__ __forceinline__ float ele_core_delta(float d2, float rinv, float ele,
 // (c p1a - c p2a d^3) / (d^9 + c p1a L), seven operations and one reciprocal
 // from d (the powers of 1/d take three more).  c p1a > 0 for every type pair
 // of the parameter table and L > 0, so the denominator is positive.
-__device__ __forceinline__ void lj_term(float d2, float d, const PairParams &q, const DevScalars &sc, Sums &s)
+__device__ __forceinline__ void lj_term(float d2, float d, const PairParams &q, const PairConst &sc, Sums &s)
 {
     const float d3 = d2 * d;
     const float d9 = (d3 * d3) * d3;
@@ -148,7 +199,7 @@ __device__ __forceinline__ void lj_term(float d2, float d, const PairParams &q, 
 // four at a time with compensation by the caller.
 template <bool HPC, bool HDB>
 __device__ __forceinline__ void pair_terms(float d2, float rinv, float hw, const PairParams &q,
-                                           const DevScalars &sc, Sums &s)
+                                           const PairConst &sc, Sums &s)
 {
     const float d = d2 * rinv;
     // hydrophobic contact inside 9 A (:44-50): with t = d2 / 81,
@@ -181,7 +232,7 @@ __device__ __forceinline__ void pair_terms(float d2, float rinv, float hw, const
 // otherwise.
 template <bool HPC, bool HDB, bool CORE = true>
 __device__ __forceinline__ void pair_block(const float4 *ptc_, const float *pw_, int pb, int pe, const Slot &me,
-                                           const PairParams &q, const DevScalars &sc, Sums &s)
+                                           const PairParams &q, const PairConst &sc, Sums &s)
 {
     const cfloat *ptc = (const cfloat *)ptc_;
     const cfloat *pw = (const cfloat *)pw_; // hydrophobic weights
@@ -246,7 +297,7 @@ __device__ __forceinline__ float dist2(const float4 &P, float qq, const Slot &me
 
 template <bool EXPAND>
 __device__ __forceinline__ void pair_block_far(const float4 *ptc_, const float *pq_, int pb, int pe, const Slot &me,
-                                               float uu, const PairParams &q, const DevScalars &sc, Sums &s)
+                                               float uu, const PairParams &q, const PairConst &sc, Sums &s)
 {
     const cfloat *ptc = (const cfloat *)ptc_;
     const cfloat *pq = (const cfloat *)pq_;
@@ -283,7 +334,7 @@ __device__ __forceinline__ void pair_block_far(const float4 *ptc_, const float *
 // LJ of one very far pair into s.v (TIER 2: its attractive part into v2);
 // returns 1/d for the electrostatics
 template <int TIER>
-__device__ __forceinline__ float vfar_lj(float d2, const PairParams &q, const DevScalars &sc, Sums &s, float &v2)
+__device__ __forceinline__ float vfar_lj(float d2, const PairParams &q, const PairConst &sc, Sums &s, float &v2)
 {
     const float rinv = __builtin_amdgcn_rsqf(d2);
     const float r2 = rinv * rinv;
@@ -310,7 +361,7 @@ __device__ __forceinline__ float vfar_lj(float d2, const PairParams &q, const De
 
 template <int TIER, bool EXPAND>
 __device__ __forceinline__ void pair_block_vfar(const float4 *ptc_, const float *pq_, int pb, int pe, const Slot &me,
-                                                float uu, const PairParams &q, const DevScalars &sc, Sums &s)
+                                                float uu, const PairParams &q, const PairConst &sc, Sums &s)
 {
     const cfloat *ptc = (const cfloat *)ptc_;
     const cfloat *pq = (const cfloat *)pq_;
@@ -382,7 +433,7 @@ size_t mc_lds_bytes(int waves, int nrep)
 // let the last round of a launch spread over every resident workgroup
 // instead of leaving CUs idle behind a partly filled round of whole tasks.
 template <bool KAHAN, int MODE, bool RECORDS>
-__global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc, const DevJob *__restrict__ jobs,
+__global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict__ prm, const DevJob *__restrict__ jobs,
                                                   const DevTask *__restrict__ tasks,
                                                   const int *__restrict__ step_base,
                                                   const int *__restrict__ pose_rep,
@@ -413,7 +464,6 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         break;
     }
     const DevTask task = tasks[ti];
-    const DevJob &J = jobs[task.job];
     const int W = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -421,7 +471,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     const int S = task.slices;
     const int sw = wave % SW;
     const int slice = wave / SW;
-    const int NL = J.natom;
+    const int NL = uref(jobs + task.job).natom;
     const int R = task.nrep;
     const int nslots = R * NL;
     const int slot = sw * 64 + lane;
@@ -443,6 +493,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     const int s1 = task.s1 + (step_base ? *step_base : 0);
 
     if (MODE == MODE_MC) {
+        const auto &J = uref(jobs + task.job);
         for (int i = threadIdx.x; i < R; i += blockDim.x) {
             const gd_ReplicaMC &rp = J.rep[task.rep0 + i];
             for (int k = 0; k < 6; ++k) st[i * 16 + k] = rp.movematrix[k];
@@ -464,25 +515,39 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         pose = task.rep0;
         rid = pose_rep[pose];
     }
-    const int idx_lig = J.rep[rid].idx_lig;
-    const uint32_t idx_rep = (uint32_t)J.rep[rid].idx_rep;
-    const float4 L3 = J.lig3[(size_t)idx_lig * NL + a];
-    const int tl = J.lt[a];
+    int idx_lig, tl;
+    uint32_t idx_rep;
+    float4 L3;
+    {
+        const auto &J = uref(jobs + task.job);
+        idx_lig = J.rep[rid].idx_lig;
+        idx_rep = (uint32_t)J.rep[rid].idx_rep;
+        L3 = J.lig3[(size_t)idx_lig * NL + a];
+        tl = J.lt[a];
+    }
     const float lc = L3.w;
     const int prt = task.prt;
-    const float4 *__restrict__ ptc0 = tg.pt + (size_t)prt * tg.npad; // ladder centre 0: the pocket-centre order
-    const int np = tg.npoint;
-    const int lo = (int)(((long long)np * slice) / S);
-    const int hi = (int)(((long long)np * (slice + 1)) / S);
-    // the segments that overlap this slice's points [lo, hi): the segments
-    // cover the points in order, so they form one run [seg_b, seg_e)
-    int seg_b = 0;
-    while (seg_b < tg.nseg && tg.seg[seg_b].z <= lo) ++seg_b;
-    int seg_e = seg_b;
-    while (seg_e < tg.nseg && tg.seg[seg_e].y < hi) ++seg_e;
+    int lo, hi, seg_b, seg_e;
+    {
+        const auto &tg = uref(prm).tg;
+        const int np = tg.npoint;
+        lo = (int)(((long long)np * slice) / S);
+        hi = (int)(((long long)np * (slice + 1)) / S);
+        // the segments that overlap this slice's points [lo, hi): the
+        // segments cover the points in order, so they form one run [seg_b, seg_e)
+        seg_b = 0;
+        while (seg_b < tg.nseg && tg.seg[seg_b].z <= lo) ++seg_b;
+        seg_e = seg_b;
+        while (seg_e < tg.nseg && tg.seg[seg_e].y < hi) ++seg_e;
+    }
 
     for (int s2 = sbeg; s2 < send; ++s2) {
         const int step = s1 + s2;
+        // the descriptors, read anew each step (see uref)
+        const auto &J = uref(jobs + task.job);
+        const auto &P = uref(prm);
+        const auto &tg = P.tg;
+        const PairConst sc{P.sc.lj_a, P.sc.lj_b, P.sc.pmf_a, P.sc.el_e0, P.sc.el_a, P.sc.el_b, P.sc.el_dc2};
         float m[6];
         float uacc = 0.0f;
         if (MODE == MODE_MC) {
@@ -514,9 +579,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // Euler rotation of the cumulative angles
         // (kernel_cpp_montecarlo_soa.cpp:241-255)
         float s1a, c1a, s2a, c2a, s3a, c3a;
-        sincosf(m[3], &s1a, &c1a);
-        sincosf(m[4], &s2a, &c2a);
-        sincosf(m[5], &s3a, &c3a);
+        sincos_cw(m[3], s1a, c1a);
+        sincos_cw(m[4], s2a, c2a);
+        sincos_cw(m[5], s3a, c3a);
         const float r00 = c1a * c2a, r01 = c1a * s2a * s3a - c3a * s1a, r02 = s1a * s3a + c1a * c3a * s2a;
         const float r10 = c2a * s1a, r11 = c1a * c3a + s1a * s2a * s3a, r12 = c3a * s1a * s2a - c1a * s3a;
         const float r20 = -s2a, r21 = c2a * s3a, r22 = c2a * c3a;
@@ -538,24 +603,30 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         const int2 kr = J.krange[a];
         const bool kde0 = task.layout & LAYOUT_KDE_SLICE0, mcs0 = task.layout & LAYOUT_MCS_SLICE0;
         if (active && (!kde0 || slice == 0)) {
+            const float kde2 = P.sc.kde2;
             const int KS = kde0 ? 1 : S;
             int k = kr.x + (kde0 ? 0 : slice);
             for (; k + KS < kr.y; k += 2 * KS) {
                 const float4 K0 = J.kde[k], K1 = J.kde[k + KS];
                 const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
                 const float bx = me.x - K1.x, by = me.y - K1.y, bz = me.z - K1.z;
-                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
-                ks += fast_exp(sc.kde2 * fmaf(bx, bx, fmaf(by, by, bz * bz)));
+                ks += fast_exp(kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
+                ks += fast_exp(kde2 * fmaf(bx, bx, fmaf(by, by, bz * bz)));
             }
             if (k < kr.y) {
                 const float4 K0 = J.kde[k];
                 const float ax = me.x - K0.x, ay = me.y - K0.y, az = me.z - K0.z;
-                ks += fast_exp(sc.kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
+                ks += fast_exp(kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
             }
         }
         if (active && (!mcs0 || slice == 0)) {
             const int MS = mcs0 ? 1 : S;
-            const float4 *lig2 = J.lig2 + (size_t)idx_lig * NL;
+            // the conformation's row offset is formed here each step from
+            // idx_lig (opaque, so the product is not kept as a 64-bit value
+            // across the step); below 2^31: n_lig x natom table entries
+            int il = idx_lig;
+            asm volatile("" : "+v"(il));
+            const float4 *lig2 = J.lig2 + il * NL;
             for (int row = a * MS + (mcs0 ? 0 : slice); row < J.nrow; row += MS * NL) {
                 const int rb = J.mrow[row], re = J.mrow[row + 1];
                 float s = 0.f;
@@ -741,6 +812,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         // PSP row add psp[t_l][residue] where d <= pmf0 (energy_prt_v1.cpp:107-109);
         // d is computed exactly as in the pair loop
         {
+            const auto &J = uref(jobs + task.job);
+            const auto &tg = uref(prm).tg;
+            const float4 *__restrict__ ptc0 = tg.pt + (size_t)prt * tg.npad; // ladder centre 0: the pocket-centre order
             float ss = 0.f;
             for (int k = 0; k < J.npsp; ++k) {
                 const int p = J.psp_pt[k];
@@ -787,18 +861,19 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
         }
         __syncthreads(); // the partials region is reused for the per-slot terms
         if (slice == 0 && active) {
+            const auto &sp = uref(prm).sc;
             // hydrophobic restraint of the atom (energy_prt_v1.cpp:154-155)
-            const float hpc2 = (ehpc - sc.hpl0[tl]) / sc.hpl1[tl];
-            const float hpc_atom = 0.5f * hpc2 * hpc2 - sc.hpl2[tl];
+            const float hpc2 = (ehpc - sp.hpl0[tl]) / sp.hpl1[tl];
+            const float hpc_atom = 0.5f * hpc2 * hpc2 - sp.hpl2[tl];
             const float kde_atom = kr.y > kr.x ? ks / (float)(kr.y - kr.x) : 0.0f;
             float *v = vals + (size_t)slot * kVals;
-            v[0] = evdw * sc.inv_lj0;
+            v[0] = evdw * sp.inv_lj0;
             // electrostatics lc / el1 (eele - cele) as hi + lo: the atoms'
             // values are up to 1e6 times their sum, so the product is split
             // exactly (FMA remainders) and both parts are summed
             {
-                const float k = lc * sc.inv_el1;
-                const float kr = fmaf(lc, sc.inv_el1, -k);
+                const float k = lc * sp.inv_el1;
+                const float kr = fmaf(lc, sp.inv_el1, -k);
                 const float hi = k * eele;
                 v[1] = hi;
                 v[8] = fmaf(k, eele, -hi) + fmaf(kr, eele, -k * cele);
@@ -824,6 +899,8 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
 
         // ---- normalise, total, accept (kernel_cpp_montecarlo_soa.cpp:333-390) ----
         for (int gg = threadIdx.x; gg < R; gg += blockDim.x) {
+            const auto &J = uref(jobs + task.job);
+            const auto &sc = uref(prm).sc;
             float e[GD_MAXWEI];
             const float *pm = prop + gg * 8;
             for (int k = 0; k < 6; ++k) e[k] = rsum[gg * 16 + k];
@@ -887,6 +964,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(DevTarget tg, DevScalars sc
     }
 
     if (MODE == MODE_MC) {
+        const auto &J = uref(jobs + task.job);
         for (int i = threadIdx.x; i < R; i += blockDim.x) {
             gd_ReplicaMC &rp = J.rep[task.rep0 + i];
             for (int k = 0; k < 6; ++k) rp.movematrix[k] = st[i * 16 + k];
@@ -968,28 +1046,27 @@ __global__ void exchange_list_kernel(const DevJob *__restrict__ jobs, const int4
 namespace {
 
 template <bool KAHAN, bool RECORDS>
-hipError_t launch_mc_impl(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
-                          const DevQueue &q, int grid, int waves, int max_rep, hipStream_t stream,
-                          const int *step_base)
+hipError_t launch_mc_impl(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q, int grid,
+                          int waves, int max_rep, hipStream_t stream, const int *step_base)
 {
     const size_t lds = mc_lds_bytes(waves, max_rep);
-    hipLaunchKernelGGL((mc_kernel<KAHAN, MODE_MC, RECORDS>), dim3(grid), dim3(waves * 64), lds, stream, tg, sc, jobs,
+    hipLaunchKernelGGL((mc_kernel<KAHAN, MODE_MC, RECORDS>), dim3(grid), dim3(waves * 64), lds, stream, prm, jobs,
                        tasks, step_base, nullptr, nullptr, nullptr, q);
     return hipGetLastError();
 }
 
-hipError_t launch_mc_any(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
-                         const DevQueue &q, int grid, int waves, int max_rep, bool compensated, bool keep_records,
-                         hipStream_t stream, const int *step_base)
+hipError_t launch_mc_any(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q, int grid,
+                         int waves, int max_rep, bool compensated, bool keep_records, hipStream_t stream,
+                         const int *step_base)
 {
     if (grid <= 0) return hipSuccess;
     if (compensated && keep_records)
-        return launch_mc_impl<true, true>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+        return launch_mc_impl<true, true>(prm, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
     if (compensated)
-        return launch_mc_impl<true, false>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+        return launch_mc_impl<true, false>(prm, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
     if (keep_records)
-        return launch_mc_impl<false, true>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
-    return launch_mc_impl<false, false>(tg, sc, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+        return launch_mc_impl<false, true>(prm, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
+    return launch_mc_impl<false, false>(prm, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
 }
 
 __global__ void queue_reset_kernel(int *counter, int *done, int n)
@@ -1001,21 +1078,19 @@ __global__ void queue_reset_kernel(int *counter, int *done, int n)
 
 } // namespace
 
-hipError_t launch_mc(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks, int ntask,
-                     int waves, int max_rep, bool compensated, bool keep_records, hipStream_t stream,
-                     const int *step_base)
+hipError_t launch_mc(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, int ntask, int waves,
+                     int max_rep, bool compensated, bool keep_records, hipStream_t stream, const int *step_base)
 {
     const DevQueue none{nullptr, nullptr, ntask, 1};
-    return launch_mc_any(tg, sc, jobs, tasks, none, ntask, waves, max_rep, compensated, keep_records, stream,
-                         step_base);
+    return launch_mc_any(prm, jobs, tasks, none, ntask, waves, max_rep, compensated, keep_records, stream, step_base);
 }
 
-hipError_t launch_mc_queue(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
-                           const DevQueue &q, int grid, int waves, int max_rep, bool compensated,
-                           bool keep_records, hipStream_t stream, const int *step_base)
+hipError_t launch_mc_queue(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q,
+                           int grid, int waves, int max_rep, bool compensated, bool keep_records,
+                           hipStream_t stream, const int *step_base)
 {
     if (!q.counter || !q.done || q.ntask <= 0 || q.nslice <= 0) return hipErrorInvalidValue;
-    return launch_mc_any(tg, sc, jobs, tasks, q, std::min(grid, q.ntask * q.nslice), waves, max_rep, compensated,
+    return launch_mc_any(prm, jobs, tasks, q, std::min(grid, q.ntask * q.nslice), waves, max_rep, compensated,
                          keep_records, stream, step_base);
 }
 
@@ -1036,18 +1111,18 @@ int mc_blocks_per_cu(int waves, int max_rep)
     return blocks;
 }
 
-hipError_t launch_energy(const DevTarget &tg, const DevScalars &sc, const DevJob *jobs, const DevTask *tasks,
-                         int ntask, int waves, bool compensated, const int *pose_rep, const float *pose_move,
-                         float *pose_out, hipStream_t stream)
+hipError_t launch_energy(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, int ntask, int waves,
+                         bool compensated, const int *pose_rep, const float *pose_move, float *pose_out,
+                         hipStream_t stream)
 {
     if (ntask <= 0) return hipSuccess;
     const size_t lds = mc_lds_bytes(waves, 1);
     const dim3 grid(ntask), block(waves * 64);
     if (compensated)
-        hipLaunchKernelGGL((mc_kernel<true, MODE_ENERGY, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
+        hipLaunchKernelGGL((mc_kernel<true, MODE_ENERGY, false>), grid, block, lds, stream, prm, jobs, tasks,
                            nullptr, pose_rep, pose_move, pose_out, DevQueue{nullptr, nullptr, ntask, 1});
     else
-        hipLaunchKernelGGL((mc_kernel<false, MODE_ENERGY, false>), grid, block, lds, stream, tg, sc, jobs, tasks,
+        hipLaunchKernelGGL((mc_kernel<false, MODE_ENERGY, false>), grid, block, lds, stream, prm, jobs, tasks,
                            nullptr, pose_rep, pose_move, pose_out, DevQueue{nullptr, nullptr, ntask, 1});
     return hipGetLastError();
 }

@@ -323,7 +323,7 @@ void gd_server::launch(int k)
         DevQueue qd{L.queue.as<int>(), L.queue.as<int>() + 1, ntask, choose_slices(ntask, slots, min_s2)};
         e = launch_queue_reset(qd, gs);
         if (e == hipSuccess)
-            e = launch_mc_queue(t.dt, t.sc, L.jobs_dev.as<DevJob>(), L.tasks_dev.as<DevTask>(), qd, slots, kServerWaves,
+            e = launch_mc_queue(t.params.as<DevParams>(), L.jobs_dev.as<DevJob>(), L.tasks_dev.as<DevTask>(), qd, slots, kServerWaves,
                                 max_rep, true, false, gs, nullptr);
     }
     for (size_t i = 0; i < L.ids.size() && e == hipSuccess; ++i) {
