@@ -18,6 +18,7 @@ INC = os.path.join(ROOT, "include")
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libgeauxdock_amd.so")
+PHASE_LIB = os.path.join(PKG, "libgeauxdock_amd_phase.so")
 APP = os.path.join(PKG, "gd_dock")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -70,6 +71,30 @@ def build(verbose=False, jobs=8):
     return LIB
 
 
+def build_phase_profile(jobs=8):
+    """The phase-profile build of the library (device code compiled with
+    -DGD_PHASE_PROFILE: clock marks per phase of the MC step, exported through
+    gd_phase_profile), beside the product library; phase_profile.py loads it.
+    The product library does not contain the marks."""
+    os.makedirs(os.path.join(BUILD, "phase"), exist_ok=True)
+    host_objs = [os.path.join(BUILD, "host", os.path.basename(s) + ".o")
+                 for s in sorted(glob.glob(os.path.join(CSRC, "host", "*.cpp")))]
+    missing = [o for o in host_objs if not os.path.exists(o)]
+    if missing:
+        build(jobs=jobs)
+    cmds, objs = [], []
+    for s in sorted(glob.glob(os.path.join(CSRC, "device", "*.hip"))):
+        o = os.path.join(BUILD, "phase", os.path.basename(s) + ".o")
+        cmds.append(DEV_FLAGS + ["-DGD_PHASE_PROFILE", "-I", os.path.join(CSRC, "device"), "-c", s, "-o", o])
+        objs.append(o)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    tmp = PHASE_LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + host_objs + objs + ["-lpthread"])
+    os.replace(tmp, PHASE_LIB)
+    return PHASE_LIB
+
+
 def build_oracle():
     """CPU oracle (test infrastructure) and, where the reference tree exists,
     the reference's own CPU kernel under oracle/_ref."""
@@ -78,5 +103,6 @@ def build_oracle():
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_phase_profile())
     build_oracle()
     sys.exit(0)

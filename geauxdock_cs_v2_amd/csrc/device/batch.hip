@@ -368,6 +368,22 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, bool compensated,
                 best = c;
             }
         }
+    // issue-priority policies of the pair loops on the chosen shape and layout
+    {
+        const Shape base = best;
+        for (int boost : {(int)LAYOUT_BOOST_ANY, (int)LAYOUT_BOOST_NONE}) {
+            Shape c = base;
+            c.layout = (base.layout & LAYOUT_ENERGY_BITS) | boost;
+            float ms = 0.f;
+            if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, large ? 1 : 2, !large, compensated, ms))
+                return rc;
+            if (int rc = bk.restore(ctx, one)) return rc;
+            if (ms < best_ms * 0.995f) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = best;
@@ -492,6 +508,20 @@ int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> 
         if (ms < best_ms * 0.995f) {
             best_ms = ms;
             best = sh;
+        }
+    }
+    // issue-priority policies of the pair loops, for all jobs at once
+    {
+        const std::vector<Shape> base = best;
+        for (int boost : {(int)LAYOUT_BOOST_ANY, (int)LAYOUT_BOOST_NONE}) {
+            std::vector<Shape> sh = base;
+            for (Shape &s : sh) s.layout = (s.layout & LAYOUT_ENERGY_BITS) | boost;
+            float ms = 0.f;
+            if (int rc = trial(sh, ms)) return rc;
+            if (ms < best_ms * 0.995f) {
+                best_ms = ms;
+                best = sh;
+            }
         }
     }
     {

@@ -166,7 +166,13 @@ struct DevTask {
 // a S + s, a S + s + S N_L, ... (every slice shares the gathers); slice 0:
 // the first point slice alone takes the atom's whole KDE run and rows
 // a, a + N_L, ... (the other slices skip them).
-enum TaskLayout { LAYOUT_KDE_SLICE0 = 1, LAYOUT_MCS_SLICE0 = 2 };
+enum TaskLayout { LAYOUT_KDE_SLICE0 = 1, LAYOUT_MCS_SLICE0 = 2, LAYOUT_BOOST_ANY = 4, LAYOUT_BOOST_NONE = 8 };
+// Issue priority of the pair loops (also autotuner candidates; the results do
+// not depend on it): a wave still in its pair loops raises its priority once
+// half of the workgroup's waves have finished theirs (default), once any has
+// (LAYOUT_BOOST_ANY), or never (LAYOUT_BOOST_NONE).
+constexpr int LAYOUT_ENERGY_BITS = LAYOUT_KDE_SLICE0 | LAYOUT_MCS_SLICE0;
+constexpr int LAYOUT_BOOST_BITS = LAYOUT_BOOST_ANY | LAYOUT_BOOST_NONE;
 
 enum KernelMode { MODE_MC = 0, MODE_ENERGY = 1 };
 

@@ -61,7 +61,8 @@ __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp
 // rad; the moves add at most rot_scale per step).  On |r| <= pi/4: Taylor
 // polynomials of sin through r^9 (next term below 2.5e-9) and of cos through
 // r^10 (below 1.2e-10); quadrant from k mod 4.  No tables and no wide-argument
-// branch, so no constants stay resident in registers across the step.
+// branch (the library routine's reduction tables held six VGPRs of
+// constants across the whole step).
 __device__ __forceinline__ void sincos_cw(float x, float &s, float &c)
 {
     const float kf = rintf(x * 0.636619772367581343f);
@@ -404,6 +405,27 @@ __device__ __forceinline__ float wave_min(float v)
 
 } // namespace
 
+#ifdef GD_PHASE_PROFILE
+// Phase profile (the library built with -DGD_PHASE_PROFILE, a tracing build
+// beside the product library): wave-cycles spent in each phase of a step,
+// each wave index's pair-loop cycles, and, per workgroup step, W times the
+// slowest wave's pair-loop cycles.  Summed in LDS per workgroup and added
+// here when the workgroup exits; read by gd_phase_profile.
+constexpr int kPhases = 9;
+__device__ unsigned long long g_phase[16];      // [kPhases] phases, [9] W x slowest, [10] workgroup steps
+__device__ unsigned long long g_phase_wave[16]; // pair-loop cycles per wave index
+#define GD_PHASE(k)                                                                                                    \
+    do {                                                                                                               \
+        const unsigned long long t_ = clock64();                                                                       \
+        if (lane == 0) atomicAdd(&s_ph[k], t_ - ph_t);                                                                 \
+        ph_t = t_;                                                                                                     \
+    } while (0)
+#else
+#define GD_PHASE(k) \
+    do {            \
+    } while (0)
+#endif
+
 size_t mc_lds_bytes(int waves, int nrep)
 {
     const int lanes = waves * 64;
@@ -442,6 +464,12 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
 {
     extern __shared__ float lds[];
     __shared__ int s_item;
+#ifdef GD_PHASE_PROFILE
+    __shared__ unsigned long long s_ph[16], s_pw[16], s_pl[16];
+    for (int i = threadIdx.x; i < 16; i += blockDim.x) s_ph[i] = s_pw[i] = 0;
+    __syncthreads();
+#endif
+    __shared__ int s_fin; // waves of the item that have finished a step's pair loops
     const int n_items = q.counter ? q.ntask * q.nslice : 0;
     for (int pass = 0;; ++pass) {
     int ti = blockIdx.x, kslice = 0, nslice = 1;
@@ -505,6 +533,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
             ist[i * kIst + 4] = 0;
         }
     }
+    if (threadIdx.x == 0) s_fin = 0;
     __syncthreads();
 
     // this lane's replica and atom
@@ -541,8 +570,24 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         while (seg_e < tg.nseg && tg.seg[seg_e].y < hi) ++seg_e;
     }
 
+    // Issue priority.  The SIMD's arbiter favours older waves, and the
+    // waves of a persistent workgroup keep their ages for the whole launch:
+    // on a SIMD shared by waves w, w + 4, w + 8 of a 12-wave workgroup, wave
+    // w finishes its pair loops first and w + 8 last, while the step's
+    // barrier waits for the last (measured on the stress shape: 4.9, 7.0 and
+    // 9.4 million cycles for the three ranks, equal work).  A wave still in
+    // its pair loops once fin_need of its siblings have finished them
+    // raises its priority (checked per segment); each step starts at 0.
+    const int fin_need = (task.layout & LAYOUT_BOOST_NONE) ? (1 << 30)
+                         : (task.layout & LAYOUT_BOOST_ANY) ? 1
+                                                            : max(1, W / 2);
     for (int s2 = sbeg; s2 < send; ++s2) {
         const int step = s1 + s2;
+#ifdef GD_PHASE_PROFILE
+        unsigned long long ph_t = clock64();
+#endif
+        __builtin_amdgcn_s_setprio(0);
+        const int fin_thr = W * (s2 - sbeg) + fin_need;
         // the descriptors, read anew each step (see uref)
         const auto &J = uref(jobs + task.job);
         const auto &P = uref(prm);
@@ -592,6 +637,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         me.z = r20 * L3.x + r21 * L3.y + r22 * L3.z + tz;
         me.tl = tl;
 
+        GD_PHASE(0); // proposal and rotation
         // KDE and MCS first: the rotation is then dead before the pair loops,
         // which need the registers.
         // KDE over the atom type's run of the type-sorted table
@@ -619,6 +665,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
                 ks += fast_exp(kde2 * fmaf(ax, ax, fmaf(ay, ay, az * az)));
             }
         }
+        GD_PHASE(1); // KDE
         if (active && (!mcs0 || slice == 0)) {
             const int MS = mcs0 ? 1 : S;
             // the conformation's row offset is formed here each step from
@@ -644,6 +691,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
             }
         }
 
+        GD_PHASE(2); // MCS
         // ---- pair terms over this slice's points ----
         float evdw = 0.f, eele = 0.f, epmf = 0.f, epsp = 0.f, ehdb = 0.f, ehpc = 0.f;
         float cvdw = 0.f, cele = 0.f, cpmf = 0.f, cpsp = 0.f, chdb = 0.f, chpc = 0.f;
@@ -704,6 +752,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
             float4 pAn = *(const float4 *)(tg.pp + ((size_t)sgn.x * GD_MAXTP2 + me.tl) * PP_N);
             for (int si = seg_b; si < seg_e; ++si) {
                 const int4 sg = sgn;
+                const int fin_now = __hip_atomic_load(&s_fin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 const float4 pA = pAn;
                 if (si + 1 < seg_e) {
                     sgn = tg.seg[si + 1];
@@ -711,6 +760,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
                 }
                 const int b = max(sg.y, lo), e = min(sg.z, hi);
                 if (b >= e) continue;
+                if (__builtin_amdgcn_readfirstlane(fin_now) >= fin_thr) __builtin_amdgcn_s_setprio(3);
                 const PairParams q{pA.x, pA.y, pA.w, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 for (int pb = b; pb < e; pb += kBlock) {
                     const int pe = min(pb + kBlock, e);
@@ -732,6 +782,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         } else {
         for (int si = seg_b; si < seg_e; ++si) {
             const int4 sg = tg.seg[si];
+            // (the count is read here and tested after the run bounds, so its
+            // LDS latency overlaps the ladder loads)
+            const int fin_now = __hip_atomic_load(&s_fin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const int b = max(sg.y, lo), e = min(sg.z, hi);
             if (b >= e) continue;
             const int ie = ki >= 0 ? ladder_in[si * kFarRungs + ki] : sg.y;
@@ -750,6 +803,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
             const int n2 = min(max(om, n1), e);
             const int mid = min(max(ob, n2), e);
             const int v2 = min(max(ovb, mid), e);
+            if (__builtin_amdgcn_readfirstlane(fin_now) >= fin_thr) __builtin_amdgcn_s_setprio(3);
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
@@ -808,6 +862,10 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
                 }
             }
         }
+#ifdef GD_PHASE_PROFILE
+        if (lane == 0) s_pl[wave] = clock64() - ph_t;
+#endif
+        GD_PHASE(3); // pair loops
         // pocket-specific term: the job's class-2 points whose residue has a
         // PSP row add psp[t_l][residue] where d <= pmf0 (energy_prt_v1.cpp:107-109);
         // d is computed exactly as in the pair loop
@@ -831,6 +889,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         }
         } // general path
 
+        if ((threadIdx.x & 63) == 0)
+            __hip_atomic_fetch_add(&s_fin, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        GD_PHASE(4); // PSP points
         // ---- cross-slice sum ----
         if (S > 1 && slice > 0) {
             float *r = red + ((size_t)(slice - 1) * SW * 64 + slot) * kRed;
@@ -860,6 +921,18 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
             }
         }
         __syncthreads(); // the partials region is reused for the per-slot terms
+#ifdef GD_PHASE_PROFILE
+        if (threadIdx.x == 0) {
+            unsigned long long mx = 0;
+            for (int w = 0; w < W; ++w) {
+                mx = s_pl[w] > mx ? s_pl[w] : mx;
+                s_pw[w] += s_pl[w];
+            }
+            s_ph[9] += mx * W;
+            s_ph[10] += 1;
+        }
+#endif
+        GD_PHASE(5); // cross-slice sum and its barriers
         if (slice == 0 && active) {
             const auto &sp = uref(prm).sc;
             // hydrophobic restraint of the atom (energy_prt_v1.cpp:154-155)
@@ -887,6 +960,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         }
         __syncthreads();
 
+        GD_PHASE(6); // per-slot terms and barrier
         // ---- per-replica sums: thread (g, k) adds term k over the atoms ----
         for (int t = threadIdx.x; t < R * kVals; t += blockDim.x) {
             const int gg = t / kVals, k = t - gg * kVals;
@@ -897,6 +971,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         }
         __syncthreads();
 
+        GD_PHASE(7); // per-replica sums and barrier
         // ---- normalise, total, accept (kernel_cpp_montecarlo_soa.cpp:333-390) ----
         for (int gg = threadIdx.x; gg < R; gg += blockDim.x) {
             const auto &J = uref(jobs + task.job);
@@ -961,6 +1036,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
             }
         }
         __syncthreads();
+        GD_PHASE(8); // accept and barrier
     }
 
     if (MODE == MODE_MC) {
@@ -985,6 +1061,13 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
     if (q.counter && threadIdx.x == 0)
         __hip_atomic_store(q.done + ti, kslice + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } // item loop
+#ifdef GD_PHASE_PROFILE
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        atomicAdd(&g_phase[threadIdx.x], s_ph[threadIdx.x]);
+        atomicAdd(&g_phase_wave[threadIdx.x], s_pw[threadIdx.x]);
+    }
+#endif
 }
 
 // Temperature replica exchange of ladder L (ligand conformation l, protein
@@ -1149,3 +1232,21 @@ hipError_t launch_exchange_list(const DevJob *jobs, const int4 *ladders, int n, 
 }
 
 } // namespace gd
+
+#ifdef GD_PHASE_PROFILE
+// Phase profile counters (out[0..15] phases, out[16..31] per wave index);
+// reset clears them.  Exported by the phase-profile build only.
+extern "C" int gd_phase_profile(unsigned long long *out, int reset)
+{
+    if (!out) return -1;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(gd::g_phase), sizeof(unsigned long long) * 16) != hipSuccess) return -3;
+    if (hipMemcpyFromSymbol(out + 16, HIP_SYMBOL(gd::g_phase_wave), sizeof(unsigned long long) * 16) != hipSuccess)
+        return -3;
+    if (reset) {
+        static const unsigned long long z[16] = {};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(gd::g_phase), z, sizeof z) != hipSuccess) return -3;
+        if (hipMemcpyToSymbol(HIP_SYMBOL(gd::g_phase_wave), z, sizeof z) != hipSuccess) return -3;
+    }
+    return 0;
+}
+#endif
