@@ -292,8 +292,18 @@ struct StateBackup {
 int trial_steps(const std::vector<JobRes *> &js, const TargetRes &t, int cap)
 {
     double pairs = 0.0;
-    for (const JobRes *j : js) pairs += (double)j->n_rep * j->natom * t.npoint;
-    return std::max(1, std::min(cap, pairs > 1e11 ? 1 : 4));
+    bool explicit_words = false;
+    for (const JobRes *j : js) {
+        pairs += (double)j->n_rep * j->natom * t.npoint;
+        explicit_words |= j->kstream_steps > 0;
+    }
+    // Small jobs run 20 steps per trial launch, so the trial splits its tasks
+    // into the step slices a 20- or 100-step chunk would (4-step trials gave
+    // c2 one step per work item, where per-item costs decide the ranking of
+    // layouts and shapes); about 10 ms per trial launch at the kernel's
+    // rate.  Jobs with explicit stream words stay within the call's steps.
+    const int want = pairs > 1e11 ? 1 : pairs > 1e10 ? 4 : 20;
+    return std::max(1, explicit_words ? std::min(cap, want) : want);
 }
 
 // ligand-size bucket of the multi-job autotuner: jobs whose atom counts fall
