@@ -2,6 +2,7 @@
 
     python -m geauxdock_cs_v2_amd.phase_profile c5 20
     python -m geauxdock_cs_v2_amd.phase_profile c2 20 --shape 12,3,4
+    python -m geauxdock_cs_v2_amd.phase_profile screen 100
 
 Loads the phase-profile build of the library (libgeauxdock_amd_phase.so,
 built by build.build_phase_profile: the product kernel with clock marks at the
@@ -26,7 +27,7 @@ PHASES = ["proposal and rotation", "KDE", "MCS", "pair loops", "PSP points", "cr
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", help="synthetic workload name (c2, c5, ...)")
+    ap.add_argument("config", help="synthetic workload name (c2, c5, ...), or screen for the 1,024-ligand batch")
     ap.add_argument("steps", type=int)
     ap.add_argument("--shape", default="", help="explicit W,SW,S (waves, slot waves, point slices)")
     args = ap.parse_args(argv)
@@ -42,21 +43,29 @@ def main(argv=None):
     if args.shape:
         w, sw, sl = map(int, args.shape.split(","))
         shape = dict(waves=w, slots_per_wave=sw, point_slices=sl)
-    cx, info = synth.load_config(args.config)
-    exch = info["cfg"]["exchange"]
     buf = (C.c_ulonglong * 32)()
     with Device(0) as dev:
-        t = dev.upload_target(cx)
-        j = dev.upload_job(t, cx, seed=2024)
-        dev.batch_run([j], 5, exchange=0, graph=True, **shape)
+        if args.config == "screen":
+            from . import perf
+            info = synth.many_files("screen")
+            cxs = synth.load_many(info, threads=perf.host_threads())
+            t = dev.upload_target(cxs[0])
+            jobs = [dev.upload_job(t, c, seed=100 + i) for i, c in enumerate(cxs)]
+            exch, warm = 0, 10
+        else:
+            cx, info = synth.load_config(args.config)
+            exch, warm = info["cfg"]["exchange"], 5
+            t = dev.upload_target(cx)
+            jobs = [dev.upload_job(t, cx, seed=2024)]
+        dev.batch_run(jobs, warm, exchange=0, graph=True, **shape)
         if lib.gd_phase_profile(buf, 1):
             raise RuntimeError("gd_phase_profile failed")
-        ms = dev.batch_run([j], args.steps, exchange=exch, graph=True, **shape)
+        ms = dev.batch_run(jobs, args.steps, exchange=exch, graph=True, **shape)
         if lib.gd_phase_profile(buf, 1):
             raise RuntimeError("gd_phase_profile failed")
-        W = dev.job_shape(j)[0]
+        W = dev.job_shape(jobs[0])[0]
         print(f"{args.config}: {args.steps} steps in {ms:.2f} ms device, shape (waves, slot waves, slices, "
-              f"replicas) {dev.job_shape(j)}")
+              f"replicas) of the first job {dev.job_shape(jobs[0])}")
     tot = sum(buf[k] for k in range(len(PHASES)))
     for k, name in enumerate(PHASES):
         print(f"  {name:32s} {100.0 * buf[k] / tot:6.2f} %")
