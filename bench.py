@@ -219,8 +219,9 @@ def run_screen(args, dev, perf, synth):
     value = n_rep * args.screen_steps / wall
     flop = sum(perf.flops_per_pose(c) * c.size["n_rep"] for c in cxs) * args.screen_steps
     tflops = flop / (dev_ms * 1e-3) / 1e12
+    waves = dev.job_shape(jobs[0])[0]
     log(f"screen batched: {args.screen_steps} steps x {n_rep} replicas in {wall:.3f} s "
-        f"({dev_ms:.1f} ms device), {value:.4g} poses/s, {tflops:.1f} TFLOP/s")
+        f"({dev_ms:.1f} ms device), {value:.4g} poses/s, {tflops:.1f} TFLOP/s, {waves}-wave workgroups")
     for j in jobs:
         dev.free_job(j)
     # plain launches, one ligand after another, on every 64th ligand
@@ -246,6 +247,7 @@ def run_screen(args, dev, perf, synth):
                         f"{cxs[0].size['prt_npoint']}-point protein x {cxs[0].size['n_prt']} conf, "
                         f"{n_rep} replicas",
             "value": value, "steps": args.screen_steps, "tflops": tflops, "frac": frac(tflops, perf),
+            "waves_per_group": waves,
             "plain": {"value": plain_value, "sample": f"ligands 0, 64, ... ({len(sample)}) x "
                                                       f"{args.screen_plain_steps} steps, one after another"},
             "speedup_vs_plain": value / plain_value}

@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
@@ -41,10 +43,15 @@ __global__ void advance_kernel(int *step_base, int dstep, int *round_base)
 std::mutex g_tune_mu;
 // single job: (N_L, N_P, n_rep, n_prt) -> shape
 std::map<std::tuple<int, int, int, int>, Shape> g_tune_cache;
-// several jobs: (N_L bucket, N_P, waves) -> shape of the bucket's jobs
-std::map<std::tuple<int, int, int>, Shape> g_bucket_cache;
-// several jobs: (N_P, bucket signature) -> waves per workgroup
-std::map<std::pair<int, std::vector<std::pair<int, int>>>, int> g_waves_cache;
+// several jobs: (N_L bucket, N_P, waves) -> slot waves of the bucket's jobs,
+// 0 where each job keeps the packing heuristic's choice for itself
+std::map<std::tuple<int, int, int>, int> g_bucket_cache;
+// several jobs: (N_P, bucket signature) -> the launch's waves per workgroup,
+// energy layout (for jobs with more than one point slice) and priority policy
+struct BatchChoice {
+    int waves, energy, boost;
+};
+std::map<std::pair<int, std::vector<std::pair<int, int>>>, BatchChoice> g_waves_cache;
 
 struct Event {
     hipEvent_t e = nullptr;
@@ -187,6 +194,14 @@ int max_rep_of(const std::vector<Shape> &shapes)
     return m;
 }
 
+// the launch runs the progress-policy kernel when every job's shape asks for it
+bool progress_of(const std::vector<Shape> &shapes)
+{
+    for (const Shape &sh : shapes)
+        if (!(sh.layout & LAYOUT_BOOST_PROGRESS)) return false;
+    return !shapes.empty();
+}
+
 // the context's work queue, with room for ntask tasks
 int ensure_queue(gd_ctx *ctx, int ntask, DevQueue &q)
 {
@@ -203,8 +218,8 @@ int ensure_queue(gd_ctx *ctx, int ntask, DevQueue &q)
 // slices chosen for s2max steps) over the work queue q (room for ntask
 // tasks): queue reset, Monte Carlo kernel.
 int enqueue_mc_on(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const DevTask *tasks_dev, int ntask,
-                  int s2max, int waves, int max_rep, bool comp, bool records, hipStream_t s, const int *step_base,
-                  DevQueue q)
+                  int s2max, int waves, int max_rep, bool comp, bool records, bool progress, hipStream_t s,
+                  const int *step_base, DevQueue q)
 {
     q.ntask = ntask;
     const int slots = resident_slots(ctx, waves, max_rep);
@@ -213,17 +228,30 @@ int enqueue_mc_on(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const
     ctx->last_plan[1] = std::min(slots, ntask * q.nslice);
     ctx->last_plan[2] = q.nslice;
     GD_HIP(launch_queue_reset(q, s));
-    GD_HIP(launch_mc_queue(t.params.as<DevParams>(), jobs_dev, tasks_dev, q, slots, waves, max_rep, comp, records, s, step_base));
+    GD_HIP(launch_mc_queue(t.params.as<DevParams>(), jobs_dev, tasks_dev, q, slots, waves, max_rep, comp, records,
+                           progress, s, step_base));
     return GD_OK;
 }
 
 // the same over the context's own work queue, grown to ntask tasks
 int enqueue_mc(gd_ctx *ctx, const TargetRes &t, const DevJob *jobs_dev, const DevTask *tasks_dev, int ntask,
-               int s2max, int waves, int max_rep, bool comp, bool records, hipStream_t s, const int *step_base)
+               int s2max, int waves, int max_rep, bool comp, bool records, bool progress, hipStream_t s,
+               const int *step_base)
 {
     DevQueue q;
     if (int rc = ensure_queue(ctx, ntask, q)) return rc;
-    return enqueue_mc_on(ctx, t, jobs_dev, tasks_dev, ntask, s2max, waves, max_rep, comp, records, s, step_base, q);
+    return enqueue_mc_on(ctx, t, jobs_dev, tasks_dev, ntask, s2max, waves, max_rep, comp, records, progress, s,
+                         step_base, q);
+}
+
+// GD_TUNE_LOG set in the environment: each autotuner trial is printed to
+// stderr (the first job's shape, the layout bits, steps and time)
+static void tune_log(const char *form, const Shape &s, int steps, float ms)
+{
+    static const bool on = std::getenv("GD_TUNE_LOG") != nullptr;
+    if (on)
+        std::fprintf(stderr, "gd tune %s: waves %d slot waves %d slices %d replicas %d layout %d, %d steps %.3f ms\n",
+                     form, s.waves, s.slot_waves, s.slices, s.nrep, s.layout, steps, ms);
 }
 
 // Time `launches` consecutive persistent launches of `steps` steps of the
@@ -239,6 +267,7 @@ int time_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js
     batch_tasks(js, shapes, s1_of, steps, tasks);
     const int W = shapes[0].waves;
     const int max_rep = max_rep_of(shapes);
+    const bool progress = progress_of(shapes);
     DevBuf tasks_dev;
     GD_HIP(upload(tasks_dev, tasks.data(), sizeof(DevTask) * tasks.size(), s));
     Event e0, e1;
@@ -246,12 +275,12 @@ int time_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js
     GD_HIP(hipEventCreate(&e1.e));
     if (warm)
         if (int rc = enqueue_mc(ctx, t, jobs_dev, tasks_dev.as<DevTask>(), (int)tasks.size(), steps, W, max_rep,
-                                comp, false, s, nullptr))
+                                comp, false, progress, s, nullptr))
             return rc;
     GD_HIP(hipEventRecord(e0.e, s));
     for (int k = 0; k < launches; ++k)
         if (int rc = enqueue_mc(ctx, t, jobs_dev, tasks_dev.as<DevTask>(), (int)tasks.size(), steps, W, max_rep,
-                                comp, false, s, nullptr))
+                                comp, false, progress, s, nullptr))
             return rc;
     GD_HIP(hipEventRecord(e1.e, s));
     GD_HIP(hipEventSynchronize(e1.e));
@@ -359,6 +388,7 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, bool compensated,
         float ms = 0.f;
         if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, large ? 1 : 2, !large, compensated, ms)) return rc;
         if (int rc = bk.restore(ctx, one)) return rc;
+        tune_log("job", c, steps, ms);
         if (ms < best_ms) {
             best_ms = ms;
             best = c;
@@ -373,23 +403,32 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, bool compensated,
             if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, large ? 1 : 2, !large, compensated, ms))
                 return rc;
             if (int rc = bk.restore(ctx, one)) return rc;
+            tune_log("job", c, steps, ms);
             if (ms < best_ms * 0.995f) { // a layout change must win by more than the noise
                 best_ms = ms;
                 best = c;
             }
         }
-    // issue-priority policies of the pair loops on the chosen shape and layout
+    // issue-priority policies of the pair loops on the chosen shape and
+    // layout, two launches each, against the shape timed again beside them
+    // (the 1-step trials of a large job vary by about 0.5 % between the
+    // shape loop and this stage)
     {
         const Shape base = best;
-        for (int boost : {(int)LAYOUT_BOOST_ANY, (int)LAYOUT_BOOST_NONE}) {
+        float ref = 0.f;
+        if (int rc = time_shapes(ctx, t, one, jobs_dev, {base}, steps, 2, !large, compensated, ref)) return rc;
+        if (int rc = bk.restore(ctx, one)) return rc;
+        tune_log("job", base, steps, ref);
+        float bar = ref * 0.995f;
+        for (int policy : {(int)LAYOUT_BOOST_ANY, (int)LAYOUT_BOOST_NONE, (int)LAYOUT_BOOST_PROGRESS}) {
             Shape c = base;
-            c.layout = (base.layout & LAYOUT_ENERGY_BITS) | boost;
+            c.layout = (base.layout & LAYOUT_ENERGY_BITS) | policy;
             float ms = 0.f;
-            if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, large ? 1 : 2, !large, compensated, ms))
-                return rc;
+            if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, 2, !large, compensated, ms)) return rc;
             if (int rc = bk.restore(ctx, one)) return rc;
-            if (ms < best_ms * 0.995f) {
-                best_ms = ms;
+            tune_log("job", c, steps, ms);
+            if (ms < bar) {
+                bar = ms;
                 best = c;
             }
         }
@@ -404,11 +443,14 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, bool compensated,
 
 // Autotuner (several jobs in one launch): the wave count W is shared by the
 // launch, the packing per ligand-size bucket.  First W from {8, 12, 16} with
-// the packing heuristic, then, bucket by bucket in decreasing share of the
-// work (buckets above 5% of it), every slot-wave count that divides W, then
-// the energy layouts for all jobs; every trial times the whole batch.
-// Results are cached per (N_P, bucket signature) for W and per (bucket, N_P,
-// W) for the packings, which later batches reuse.
+// the packing heuristic (each job's own choice), then, bucket by bucket in
+// decreasing share of the work (buckets above 5% of it), every slot-wave
+// count that divides W for all of the bucket's jobs, then the energy layouts
+// and the priority policies for all jobs; every trial times the whole batch.
+// Results are cached per (N_P, bucket signature) for W, layout and policy and
+// per (bucket, N_P, W) for the packings (a bucket no trial changed keeps the
+// heuristic's per-job choice), so a later batch runs the shapes the trials
+// chose.
 int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js, const DevJob *jobs_dev,
                    bool compensated, int max_steps, std::vector<Shape> &shapes)
 {
@@ -438,7 +480,8 @@ int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> 
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto wit = g_waves_cache.find({t.npoint, sig});
         if (wit != g_waves_cache.end()) {
-            const int W = wit->second;
+            const BatchChoice ch = wit->second;
+            const int W = ch.waves;
             std::vector<Shape> sh;
             heuristic_all(W, sh);
             bool all = true;
@@ -447,8 +490,8 @@ int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> 
                 if (b == g_bucket_cache.end()) {
                     all = false;
                 } else {
-                    sh[i] = packed_shape(js[i]->natom, W, b->second.slot_waves, js[i]->n_tmp * js[i]->n_lig);
-                    sh[i].layout = b->second.layout;
+                    if (b->second > 0) sh[i] = packed_shape(js[i]->natom, W, b->second, js[i]->n_tmp * js[i]->n_lig);
+                    sh[i].layout = (sh[i].slices > 1 ? ch.energy : 0) | ch.boost;
                 }
             }
             if (all) {
@@ -464,24 +507,31 @@ int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> 
         float a = 0.f;
         if (int rc = time_shapes(ctx, t, js, jobs_dev, sh, steps, 1, true, compensated, a)) return rc;
         if (int rc = bk.restore(ctx, js)) return rc;
+        tune_log("batch", sh[0], steps, a);
         ms = a;
         return GD_OK;
     };
     float best_ms = 1e30f;
     std::vector<Shape> best;
+    // a larger W must win by more than the noise: on near-ties the smaller
+    // workgroup (fewer waves per SIMD at the step's barriers) measured faster
+    // over long runs (the 1,024-ligand screen: 8 and 12 waves within 0.3 % in
+    // 4-step trials, 92.8 and 88.4 TFLOP/s over 100 steps)
     for (int W : {8, 12, 16}) {
         if (max_natom > 64 * W) continue;
         std::vector<Shape> sh;
         heuristic_all(W, sh);
         float ms = 0.f;
         if (int rc = trial(sh, ms)) return rc;
-        if (ms < best_ms) {
+        if (ms < (best.empty() ? 1e30f : best_ms * 0.995f)) {
             best_ms = ms;
             best = sh;
         }
     }
     if (best.empty()) return fail(GD_ERR_ARG, "ligand too large for a 16-wave workgroup");
     const int W = best[0].waves;
+    std::map<int, int> bucket_sw; // buckets whose slot-wave count a trial chose
+    int energy = 0, boost = 0;
     std::vector<std::pair<double, int>> order;
     for (const auto &kv : work) order.emplace_back(-kv.second, kv.first);
     std::sort(order.begin(), order.end());
@@ -507,6 +557,7 @@ int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> 
             if (ms < best_ms * 0.995f) {
                 best_ms = ms;
                 best = sh;
+                bucket_sw[b] = SW;
             }
         }
     }
@@ -518,26 +569,35 @@ int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> 
         if (ms < best_ms * 0.995f) {
             best_ms = ms;
             best = sh;
+            energy = layout;
         }
     }
-    // issue-priority policies of the pair loops, for all jobs at once
+    // issue-priority policies of the pair loops, for all jobs at once,
+    // against the shapes timed again beside them
     {
         const std::vector<Shape> base = best;
-        for (int boost : {(int)LAYOUT_BOOST_ANY, (int)LAYOUT_BOOST_NONE}) {
+        float ref = 0.f;
+        if (int rc = trial(base, ref)) return rc;
+        float bar = ref * 0.995f;
+        for (int policy : {(int)LAYOUT_BOOST_ANY, (int)LAYOUT_BOOST_NONE, (int)LAYOUT_BOOST_PROGRESS}) {
             std::vector<Shape> sh = base;
-            for (Shape &s : sh) s.layout = (s.layout & LAYOUT_ENERGY_BITS) | boost;
+            for (Shape &s : sh) s.layout = (s.layout & LAYOUT_ENERGY_BITS) | policy;
             float ms = 0.f;
             if (int rc = trial(sh, ms)) return rc;
-            if (ms < best_ms * 0.995f) {
-                best_ms = ms;
+            if (ms < bar) {
+                bar = ms;
                 best = sh;
+                boost = policy;
             }
         }
     }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_waves_cache[{t.npoint, sig}] = W;
-        for (size_t i = 0; i < n; ++i) g_bucket_cache[std::make_tuple(bucket[i], t.npoint, W)] = best[i];
+        g_waves_cache[{t.npoint, sig}] = BatchChoice{W, energy, boost};
+        for (size_t i = 0; i < n; ++i) {
+            const auto it = bucket_sw.find(bucket[i]);
+            g_bucket_cache[std::make_tuple(bucket[i], t.npoint, W)] = it == bucket_sw.end() ? 0 : it->second;
+        }
     }
     shapes = best;
     return GD_OK;
@@ -600,8 +660,8 @@ int batch_chunk(const gd_batch_opts *opts)
 // Workgroup shapes: explicit in opts, the packing heuristic for a given wave
 // count, or (no wave count) autotuned when `tune`, else the autotuner's
 // cached choice where there is one and the heuristic at 12 waves otherwise.
-int batch_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js, const gd_batch_opts *opts,
-                 const DevJob *jobs_dev, bool tune, std::vector<Shape> &shapes)
+int choose_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js, const gd_batch_opts *opts,
+                  const DevJob *jobs_dev, bool tune, std::vector<Shape> &shapes)
 {
     const int n_jobs = (int)js.size();
     const bool comp = opts->compensated != 0;
@@ -643,6 +703,31 @@ int batch_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &j
     return GD_OK;
 }
 
+// GD_BOOST_POLICY in the environment (half, any, none or progress) sets the
+// issue-priority policy of every job's shape, whatever chose the shape (the
+// autotuner's cache keeps its own choice); unset or another value: no change.
+int boost_override()
+{
+    const char *v = std::getenv("GD_BOOST_POLICY");
+    if (!v) return -1;
+    const std::string p(v);
+    return p == "half"       ? 0
+           : p == "any"      ? (int)LAYOUT_BOOST_ANY
+           : p == "none"     ? (int)LAYOUT_BOOST_NONE
+           : p == "progress" ? (int)LAYOUT_BOOST_PROGRESS
+                             : -1;
+}
+
+int batch_shapes(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> &js, const gd_batch_opts *opts,
+                 const DevJob *jobs_dev, bool tune, std::vector<Shape> &shapes)
+{
+    if (int rc = choose_shapes(ctx, t, js, opts, jobs_dev, tune, shapes)) return rc;
+    const int boost = boost_override();
+    if (boost >= 0)
+        for (Shape &sh : shapes) sh.layout = (sh.layout & LAYOUT_ENERGY_BITS) | boost;
+    return GD_OK;
+}
+
 // Device memory a batched run uses beyond the jobs' own tables.
 struct BatchMem {
     DevJob *jobs = nullptr;              // [n_jobs] job descriptors
@@ -651,6 +736,7 @@ struct BatchMem {
     DevQueue queue{nullptr, nullptr, 0, 1}; // counter and done[ntask]
     int4 *ladders = nullptr;             // [nlad] exchange ladders
     int *bases = nullptr;                // [2] step and exchange-round counters
+    bool progress = false;               // the progress-policy kernel (progress_of)
 };
 
 // The chunk sequence of a batched run, once its tables are in `mem`: per
@@ -663,8 +749,8 @@ int batch_launches(gd_ctx *ctx, const TargetRes &t, const gd_batch_opts *opts, i
     hipStream_t s = ctx->stream;
     const bool comp = opts->compensated != 0, records = opts->keep_records != 0;
     auto enqueue_period = [&]() -> int {
-        if (int rc = enqueue_mc_on(ctx, t, mem.jobs, mem.tasks, ntask, chunk, W, max_rep, comp, records, s, mem.bases,
-                                   mem.queue))
+        if (int rc = enqueue_mc_on(ctx, t, mem.jobs, mem.tasks, ntask, chunk, W, max_rep, comp, records,
+                                   mem.progress, s, mem.bases, mem.queue))
             return rc;
         if (nlad > 0) GD_HIP(launch_exchange_list(mem.jobs, mem.ladders, nlad, s, mem.bases + 1));
         hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, mem.bases, chunk, mem.bases + 1);
@@ -696,8 +782,8 @@ int batch_launches(gd_ctx *ctx, const TargetRes &t, const gd_batch_opts *opts, i
             return rc;
     }
     if (rest > 0)
-        if (int rc = enqueue_mc_on(ctx, t, mem.jobs, mem.tail, ntask, rest, W, max_rep, comp, records, s, mem.bases,
-                                   mem.queue))
+        if (int rc = enqueue_mc_on(ctx, t, mem.jobs, mem.tail, ntask, rest, W, max_rep, comp, records,
+                                   mem.progress, s, mem.bases, mem.queue))
             return rc;
     float ms = 0.f;
     if (use_graph) {
@@ -790,6 +876,7 @@ extern "C" int gd_batch_run(gd_ctx *ctx, const int *job_ids, int n_jobs, const g
     mem.tail = tail.empty() ? nullptr : tail_dev.as<DevTask>();
     mem.ladders = ctx->ladders.as<int4>();
     mem.bases = ctx->bases.as<int>();
+    mem.progress = progress_of(shapes);
     if (int rc = batch_launches(ctx, t, opts, chunk, W, max_rep, ntask, (int)lad.size(), mem, opts->use_graph != 0,
                                 elapsed_ms))
         return rc;
@@ -885,6 +972,7 @@ extern "C" int gd_batch_run_noalloc(gd_ctx *ctx, const int *job_ids, int n_jobs,
     mem.tasks = (DevTask *)(b + o_tasks);
     mem.tail = tail.empty() ? nullptr : (DevTask *)(b + o_tail);
     mem.ladders = (int4 *)(b + o_lad);
+    mem.progress = progress_of(shapes);
     // no graph: instantiating one may allocate
     if (int rc = batch_launches(ctx, t, opts, chunk, W, max_rep, (int)ntask, (int)lad.size(), mem, false, elapsed_ms))
         return rc;

@@ -166,13 +166,21 @@ struct DevTask {
 // a S + s, a S + s + S N_L, ... (every slice shares the gathers); slice 0:
 // the first point slice alone takes the atom's whole KDE run and rows
 // a, a + N_L, ... (the other slices skip them).
-enum TaskLayout { LAYOUT_KDE_SLICE0 = 1, LAYOUT_MCS_SLICE0 = 2, LAYOUT_BOOST_ANY = 4, LAYOUT_BOOST_NONE = 8 };
+enum TaskLayout {
+    LAYOUT_KDE_SLICE0 = 1,
+    LAYOUT_MCS_SLICE0 = 2,
+    LAYOUT_BOOST_ANY = 4,
+    LAYOUT_BOOST_NONE = 8,
+    LAYOUT_BOOST_PROGRESS = 16
+};
 // Issue priority of the pair loops (also autotuner candidates; the results do
 // not depend on it): a wave still in its pair loops raises its priority once
 // half of the workgroup's waves have finished theirs (default), once any has
-// (LAYOUT_BOOST_ANY), or never (LAYOUT_BOOST_NONE).
+// (LAYOUT_BOOST_ANY), or never (LAYOUT_BOOST_NONE); or, in a launch whose
+// tasks all carry LAYOUT_BOOST_PROGRESS, a wave takes priority while it has
+// started fewer of its points than the workgroup's mean (a kernel of its own).
 constexpr int LAYOUT_ENERGY_BITS = LAYOUT_KDE_SLICE0 | LAYOUT_MCS_SLICE0;
-constexpr int LAYOUT_BOOST_BITS = LAYOUT_BOOST_ANY | LAYOUT_BOOST_NONE;
+constexpr int LAYOUT_BOOST_BITS = LAYOUT_BOOST_ANY | LAYOUT_BOOST_NONE | LAYOUT_BOOST_PROGRESS;
 
 enum KernelMode { MODE_MC = 0, MODE_ENERGY = 1 };
 
@@ -199,9 +207,11 @@ hipError_t launch_mc(const DevParams *prm, const DevJob *jobs, const DevTask *ta
 // Persistent form: `grid` workgroups walk the queue q (q.ntask tasks in
 // q.nslice step slices each).  The queue's counter and done[] must be zero
 // at launch (launch_queue_reset clears them on the stream, with a kernel, so
-// the sequence can be captured in a graph).
+// the sequence can be captured in a graph).  progress: the kernel built with
+// the progress-balancing issue-priority policy (LAYOUT_BOOST_PROGRESS; used
+// with compensated sums and no records, else ignored).
 hipError_t launch_mc_queue(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q,
-                           int grid, int waves, int max_rep, bool compensated, bool keep_records,
+                           int grid, int waves, int max_rep, bool compensated, bool keep_records, bool progress,
                            hipStream_t stream, const int *step_base);
 hipError_t launch_queue_reset(const DevQueue &q, hipStream_t stream);
 

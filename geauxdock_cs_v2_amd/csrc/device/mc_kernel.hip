@@ -436,10 +436,34 @@ size_t mc_lds_bytes(int waves, int nrep)
     return sizeof(float) * ((size_t)lanes * kVals + (size_t)nrep * 40) + sizeof(int) * (size_t)nrep * kIst;
 }
 
+// Issue priority at a segment of the pair loops.  Without PROG: priority 3
+// once the count of siblings done with the step's pair loops (fin_now, read
+// at the segment's start) reaches fin_thr.  With PROG: the wave adds its
+// segment's n points to the workgroup's count and takes priority 3 while
+// W x its own points started before the segment (own) is below the count
+// read at the segment's start (unsigned arithmetic: the difference is exact
+// while the waves stay within 2^31 points of each other), else priority 0.
+template <bool PROG>
+__device__ __forceinline__ void seg_priority(int *fin, int fin_now, int fin_thr, int W, unsigned own, int n)
+{
+    if constexpr (PROG) {
+        if ((threadIdx.x & 63) == 0)
+            __hip_atomic_fetch_add(fin, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((int)((unsigned)__builtin_amdgcn_readfirstlane(fin_now) - (unsigned)W * own) > 0)
+            __builtin_amdgcn_s_setprio(3);
+        else
+            __builtin_amdgcn_s_setprio(0);
+    } else {
+        if (__builtin_amdgcn_readfirstlane(fin_now) >= fin_thr) __builtin_amdgcn_s_setprio(3);
+    }
+}
+
 // At least 6 waves per SIMD (at most 80 VGPRs): two 12-wave workgroups share
 // a CU.  RECORDS: append every accepted step to the job's record window (the
 // plain chunk keeps them; batched runs usually do not, and the
-// instantiations have distinct names in profiles).
+// instantiations have distinct names in profiles).  PROG: the issue-priority
+// policy that balances the waves' progress through the pair loops (below),
+// instead of the policy the task's layout bits select.
 //
 // Work distribution.  Without a queue (q.counter null) workgroup b runs task
 // b for all of its steps.  With a queue the grid is the number of
@@ -454,7 +478,7 @@ size_t mc_lds_bytes(int waves, int nrep)
 // which itself waits only on earlier items, so the waits end.  Step slices
 // let the last round of a launch spread over every resident workgroup
 // instead of leaving CUs idle behind a partly filled round of whole tasks.
-template <bool KAHAN, int MODE, bool RECORDS>
+template <bool KAHAN, int MODE, bool RECORDS, bool PROG = false>
 __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict__ prm, const DevJob *__restrict__ jobs,
                                                   const DevTask *__restrict__ tasks,
                                                   const int *__restrict__ step_base,
@@ -469,7 +493,9 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
     for (int i = threadIdx.x; i < 16; i += blockDim.x) s_ph[i] = s_pw[i] = 0;
     __syncthreads();
 #endif
-    __shared__ int s_fin; // waves of the item that have finished a step's pair loops
+    // waves of the item that have finished a step's pair loops; with PROG
+    // the points the item's waves have started in them (modulo 2^32)
+    __shared__ int s_fin;
     const int n_items = q.counter ? q.ntask * q.nslice : 0;
     for (int pass = 0;; ++pass) {
     int ti = blockIdx.x, kslice = 0, nslice = 1;
@@ -578,6 +604,10 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
     // 9.4 million cycles for the three ranks, equal work).  A wave still in
     // its pair loops once fin_need of its siblings have finished them
     // raises its priority (checked per segment); each step starts at 0.
+    // PROG instead: at each segment a wave adds the segment's points to the
+    // count and takes priority 3 while it has started fewer points than the
+    // workgroup's mean, else 0 (every wave walks a slice of the same size,
+    // so equal counts mean equal shares of the step done).
     const int fin_need = (task.layout & LAYOUT_BOOST_NONE) ? (1 << 30)
                          : (task.layout & LAYOUT_BOOST_ANY) ? 1
                                                             : max(1, W / 2);
@@ -588,6 +618,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
 #endif
         __builtin_amdgcn_s_setprio(0);
         const int fin_thr = W * (s2 - sbeg) + fin_need;
+        const unsigned own0 = (unsigned)(s2 - sbeg) * (unsigned)(hi - lo);
         // the descriptors, read anew each step (see uref)
         const auto &J = uref(jobs + task.job);
         const auto &P = uref(prm);
@@ -760,7 +791,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
                 }
                 const int b = max(sg.y, lo), e = min(sg.z, hi);
                 if (b >= e) continue;
-                if (__builtin_amdgcn_readfirstlane(fin_now) >= fin_thr) __builtin_amdgcn_s_setprio(3);
+                seg_priority<PROG>(&s_fin, fin_now, fin_thr, W, own0 + (unsigned)(b - lo), e - b);
                 const PairParams q{pA.x, pA.y, pA.w, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 for (int pb = b; pb < e; pb += kBlock) {
                     const int pe = min(pb + kBlock, e);
@@ -803,7 +834,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
             const int n2 = min(max(om, n1), e);
             const int mid = min(max(ob, n2), e);
             const int v2 = min(max(ovb, mid), e);
-            if (__builtin_amdgcn_readfirstlane(fin_now) >= fin_thr) __builtin_amdgcn_s_setprio(3);
+            seg_priority<PROG>(&s_fin, fin_now, fin_thr, W, own0 + (unsigned)(b - lo), e - b);
             const float *P = tg.pp + ((size_t)sg.x * GD_MAXTP2 + me.tl) * PP_N;
             const float4 pA = *(const float4 *)P;
             const float4 pB = *(const float4 *)(P + 4);
@@ -889,7 +920,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         }
         } // general path
 
-        if ((threadIdx.x & 63) == 0)
+        if (!PROG && (threadIdx.x & 63) == 0)
             __hip_atomic_fetch_add(&s_fin, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         GD_PHASE(4); // PSP points
         // ---- cross-slice sum ----
@@ -1128,21 +1159,25 @@ __global__ void exchange_list_kernel(const DevJob *__restrict__ jobs, const int4
 
 namespace {
 
-template <bool KAHAN, bool RECORDS>
+template <bool KAHAN, bool RECORDS, bool PROG = false>
 hipError_t launch_mc_impl(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q, int grid,
                           int waves, int max_rep, hipStream_t stream, const int *step_base)
 {
     const size_t lds = mc_lds_bytes(waves, max_rep);
-    hipLaunchKernelGGL((mc_kernel<KAHAN, MODE_MC, RECORDS>), dim3(grid), dim3(waves * 64), lds, stream, prm, jobs,
-                       tasks, step_base, nullptr, nullptr, nullptr, q);
+    hipLaunchKernelGGL((mc_kernel<KAHAN, MODE_MC, RECORDS, PROG>), dim3(grid), dim3(waves * 64), lds, stream, prm,
+                       jobs, tasks, step_base, nullptr, nullptr, nullptr, q);
     return hipGetLastError();
 }
 
+// the progress policy is built for the batched form's kernel (compensated
+// sums, no records); the other forms keep the layout bits' policy
 hipError_t launch_mc_any(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q, int grid,
-                         int waves, int max_rep, bool compensated, bool keep_records, hipStream_t stream,
-                         const int *step_base)
+                         int waves, int max_rep, bool compensated, bool keep_records, bool progress,
+                         hipStream_t stream, const int *step_base)
 {
     if (grid <= 0) return hipSuccess;
+    if (compensated && !keep_records && progress)
+        return launch_mc_impl<true, false, true>(prm, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
     if (compensated && keep_records)
         return launch_mc_impl<true, true>(prm, jobs, tasks, q, grid, waves, max_rep, stream, step_base);
     if (compensated)
@@ -1165,16 +1200,17 @@ hipError_t launch_mc(const DevParams *prm, const DevJob *jobs, const DevTask *ta
                      int max_rep, bool compensated, bool keep_records, hipStream_t stream, const int *step_base)
 {
     const DevQueue none{nullptr, nullptr, ntask, 1};
-    return launch_mc_any(prm, jobs, tasks, none, ntask, waves, max_rep, compensated, keep_records, stream, step_base);
+    return launch_mc_any(prm, jobs, tasks, none, ntask, waves, max_rep, compensated, keep_records, false, stream,
+                         step_base);
 }
 
 hipError_t launch_mc_queue(const DevParams *prm, const DevJob *jobs, const DevTask *tasks, const DevQueue &q,
-                           int grid, int waves, int max_rep, bool compensated, bool keep_records,
+                           int grid, int waves, int max_rep, bool compensated, bool keep_records, bool progress,
                            hipStream_t stream, const int *step_base)
 {
     if (!q.counter || !q.done || q.ntask <= 0 || q.nslice <= 0) return hipErrorInvalidValue;
     return launch_mc_any(prm, jobs, tasks, q, std::min(grid, q.ntask * q.nslice), waves, max_rep, compensated,
-                         keep_records, stream, step_base);
+                         keep_records, progress, stream, step_base);
 }
 
 hipError_t launch_queue_reset(const DevQueue &q, hipStream_t stream)

@@ -324,7 +324,7 @@ void gd_server::launch(int k)
         e = launch_queue_reset(qd, gs);
         if (e == hipSuccess)
             e = launch_mc_queue(t.params.as<DevParams>(), L.jobs_dev.as<DevJob>(), L.tasks_dev.as<DevTask>(), qd, slots, kServerWaves,
-                                max_rep, true, false, gs, nullptr);
+                                max_rep, true, false, false, gs, nullptr);
     }
     for (size_t i = 0; i < L.ids.size() && e == hipSuccess; ++i) {
         if (!L.exch[i]) continue;

@@ -397,6 +397,33 @@ def test_graph_replay_is_bit_identical(lib, ctx):
     assert res[0] == res[1]
 
 
+@pytest.mark.parametrize("slices", [1, 4])
+def test_issue_priority_policies_are_bit_identical(lib, ctx, slices):
+    """The pair loops' issue-priority policies (GD_BOOST_POLICY: after half
+    of the siblings, after any, never, and the progress-balancing kernel of
+    its own) change only which wave issues first: 20 compensated steps with
+    an exchange give the same replica states, byte for byte."""
+    from geauxdock_cs_v2_amd import synth
+    cx, _ = synth.load_config("c2")
+    n = cx.size["n_rep"]
+    res = {}
+    try:
+        for policy in ("half", "any", "none", "progress"):
+            os.environ["GD_BOOST_POLICY"] = policy
+            say(f"case: c2 batched 20 steps, 12 waves x {slices} slices, policy {policy}")
+            t, j = upload(lib, ctx, cx, 31)
+            o = D.BatchOpts(20, 10, 0, 0, 12, slices, 1, 0)
+            arr = (C.c_int * 1)(j)
+            ms = C.c_double()
+            L.check(lib.gd_batch_run(ctx, arr, 1, C.byref(o), C.byref(ms)))
+            res[policy] = bytes(read_reps(lib, ctx, j, n))
+            free(lib, ctx, t, j)
+    finally:
+        os.environ.pop("GD_BOOST_POLICY", None)
+    for policy in ("any", "none", "progress"):
+        assert res[policy] == res["half"], policy
+
+
 def test_compensated_accumulation_within_tolerance(lib, ctx):
     """Kahan-compensated FP32 accumulation: 20 steps, final energies
     re-evaluated by the FP64 oracle at the GPU's final moves."""
