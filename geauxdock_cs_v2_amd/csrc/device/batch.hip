@@ -410,13 +410,15 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, bool compensated,
             }
         }
     // issue-priority policies of the pair loops on the chosen shape and
-    // layout, two launches each, against the shape timed again beside them
-    // (the 1-step trials of a large job vary by about 0.5 % between the
-    // shape loop and this stage)
+    // layout, against the shape timed again beside them (the 1-step trials of
+    // a large job vary by about 0.5 % between the shape loop and this stage,
+    // and the policies differ by about 1 %): two launches each, four of a
+    // large job's one step
     {
         const Shape base = best;
+        const int reps = large ? 4 : 2;
         float ref = 0.f;
-        if (int rc = time_shapes(ctx, t, one, jobs_dev, {base}, steps, 2, !large, compensated, ref)) return rc;
+        if (int rc = time_shapes(ctx, t, one, jobs_dev, {base}, steps, reps, !large, compensated, ref)) return rc;
         if (int rc = bk.restore(ctx, one)) return rc;
         tune_log("job", base, steps, ref);
         float bar = ref * 0.995f;
@@ -424,7 +426,7 @@ int autotune_shape(gd_ctx *ctx, const TargetRes &t, JobRes &j, bool compensated,
             Shape c = base;
             c.layout = (base.layout & LAYOUT_ENERGY_BITS) | policy;
             float ms = 0.f;
-            if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, 2, !large, compensated, ms)) return rc;
+            if (int rc = time_shapes(ctx, t, one, jobs_dev, {c}, steps, reps, !large, compensated, ms)) return rc;
             if (int rc = bk.restore(ctx, one)) return rc;
             tune_log("job", c, steps, ms);
             if (ms < bar) {
