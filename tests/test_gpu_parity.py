@@ -424,6 +424,44 @@ def test_issue_priority_policies_are_bit_identical(lib, ctx, slices):
         assert res[policy] == res["half"], policy
 
 
+def test_cached_batch_shapes_match_tuned(lib, ctx):
+    """A batch that finds the autotuner's choice in its cache runs the shapes
+    the trials timed.  The first 16 screening ligands (16-79 atoms) are a
+    batch whose packing heuristic picks different slot-wave counts for jobs
+    of one ligand-size bucket (24 and 48 atoms), which a per-bucket cache of
+    one job's packing would replace with that job's."""
+    from geauxdock_cs_v2_amd import synth
+    info = dict(synth.many_files("screen"))
+    info["ligands"] = info["ligands"][:16]
+    cxs = synth.load_many(info, threads=8)
+    t = C.c_int()
+    L.check(lib.gd_upload_target(ctx, C.byref(cxs[0].view), C.byref(t)))
+    jobs = []
+    for i, cx in enumerate(cxs):
+        j = C.c_int()
+        L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), 500 + i, C.byref(j)))
+        jobs.append(j.value)
+
+    def shapes():
+        out = []
+        for j in jobs:
+            s = (C.c_int * 4)()
+            L.check(lib.gd_job_shape(ctx, j, s))
+            out.append(tuple(s))
+        return out
+
+    say("case: 16 screening ligands, autotuned batch then the cached choice")
+    _batch(lib, ctx, jobs, 4, comp=1)
+    tuned = shapes()
+    _batch(lib, ctx, jobs, 4, comp=1)
+    cached = shapes()
+    for j in jobs:
+        L.check(lib.gd_free_job(ctx, j))
+    L.check(lib.gd_free_target(ctx, t))
+    say("tuned slot waves:", [s[1] for s in tuned])
+    assert cached == tuned
+
+
 def test_compensated_accumulation_within_tolerance(lib, ctx):
     """Kahan-compensated FP32 accumulation: 20 steps, final energies
     re-evaluated by the FP64 oracle at the GPU's final moves."""
