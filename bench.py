@@ -121,6 +121,26 @@ def traffic_from_summary(steps_per_launch):
     return b * steps_per_launch / s, s
 
 
+def issue_from_summary():
+    """VALU issue figures of the headline kernel from the committed rocprofv3
+    instruction-counter summary (the same file as the traffic), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            text = f.read()
+    except OSError:
+        return None
+    out = {}
+    for key in ("valu_per_wave_pair", "trans_per_wave_pair", "valu_issue_per_quad_cycle"):
+        m = re.search(rf"^{key}:\s*([0-9.eE+]+)", text, re.M)
+        if not m:
+            return None
+        out[key] = float(m.group(1))
+    out["trans_share"] = out["trans_per_wave_pair"] / out["valu_per_wave_pair"]
+    out["issue_peak_per_quad_cycle"] = 2.0
+    out["source"] = os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+    return out
+
+
 def batched_window(dev, jobs, steps, warmup, exchange, shape=None):
     """Warm-up (autotuner trials and `warmup` steps), then `steps` timed
     steps; returns (wall s, device ms, launches, steps per launch)."""
@@ -422,6 +442,9 @@ def main():
             roof["hbm_frac"] = roof["hbm_gbs"] / perf.HBM_PEAK_GBS
             roof["traffic_source"] = os.path.relpath(TRAFFIC_SUMMARY, ROOT)
             roof["traffic_measured_steps_per_launch"] = traffic_steps
+        issue = issue_from_summary()
+        if issue:
+            roof["issue"] = issue
     head = stress or {"value": None, "wall_s": float("nan"), "workload": None, "replicas": None,
                       "plain": {"value": None}, "speedup_vs_plain": None}
     out = {
