@@ -203,12 +203,14 @@ def run_ensemble(args, dev, perf, synth, name, steps, warmup, seed):
     j = dev.upload_job(t, cx, seed=seed)
     wall, dev_ms, launches, spl = batched_window(dev, [j], steps, warmup, exch)
     shape = dev.job_shape(j)
+    plan = dev.batch_plan()  # of the window's last launch
     dev.free_job(j)
     value = n_rep * steps / wall
     launch_ms = dev_ms / launches
     tflops = f_pose * n_rep * spl / (launch_ms * 1e-3) / 1e12
     log(f"{name} batched: {steps} steps in {wall:.3f} s wall ({dev_ms:.1f} ms device, {launches} launches of "
-        f"{launch_ms:.2f} ms), {value:.4g} poses/s, {tflops:.1f} TFLOP/s")
+        f"{launch_ms:.2f} ms), {value:.4g} poses/s, {tflops:.1f} TFLOP/s; {plan[0]} tasks in {plan[2]} step "
+        f"slices over {plan[1]} workgroups")
     plain = None
     if args.plain:
         plain = plain_window(dev, t, cx, steps, exch, seed)
@@ -219,6 +221,7 @@ def run_ensemble(args, dev, perf, synth, name, steps, warmup, seed):
                 "launch_ms": launch_ms, "flop_per_pose": f_pose, "tflops": tflops, "frac": frac(tflops, perf),
                 "shape": {"waves": shape[0], "slot_waves": shape[1], "point_slices": shape[2],
                           "replicas_per_group": shape[3], "chosen_by": "autotuner"},
+                "plan": {"tasks": plan[0], "workgroups": plan[1], "step_slices": plan[2]},
                 "plain": {"value": plain, "steps": steps,
                           "form": "one workgroup per replica, one launch per chunk, record read-back per chunk"},
                 "speedup_vs_plain": value / plain if plain else None}
@@ -436,7 +439,7 @@ def main():
                 "steps_per_launch": stress["steps_per_launch"],
                 "unit_of_work": f"one launch of the Monte Carlo kernel: {stress['steps_per_launch']:g} steps of all "
                                 f"{stress['replicas']} replicas",
-                "flop_per_pose": stress["flop_per_pose"], "shape": stress["shape"]}
+                "flop_per_pose": stress["flop_per_pose"], "shape": stress["shape"], "plan": stress["plan"]}
         if traffic:
             roof["hbm_gbs"] = traffic / (stress["launch_ms"] * 1e-3) / 1e9
             roof["hbm_frac"] = roof["hbm_gbs"] / perf.HBM_PEAK_GBS
