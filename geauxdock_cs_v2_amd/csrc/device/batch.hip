@@ -515,10 +515,10 @@ int autotune_batch(gd_ctx *ctx, const TargetRes &t, const std::vector<JobRes *> 
     };
     float best_ms = 1e30f;
     std::vector<Shape> best;
-    // a larger W must win by more than the noise: on near-ties the smaller
-    // workgroup (fewer waves per SIMD at the step's barriers) measured faster
-    // over long runs (the 1,024-ligand screen: 8 and 12 waves within 0.3 % in
-    // 4-step trials, 92.8 and 88.4 TFLOP/s over 100 steps)
+    // a larger W must win by more than the noise (0.5 %): on near-ties the
+    // smaller workgroup, with fewer waves per SIMD waiting at the step's
+    // barriers, is kept (the 1,024-ligand screen: 8 and 12 waves within 0.3 %
+    // in 4-step trials and 95.5 against 95.3 TFLOP/s over 100 steps)
     for (int W : {8, 12, 16}) {
         if (max_natom > 64 * W) continue;
         std::vector<Shape> sh;
