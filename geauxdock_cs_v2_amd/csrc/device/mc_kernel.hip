@@ -414,6 +414,10 @@ __device__ __forceinline__ float wave_min(float v)
 constexpr int kPhases = 9;
 __device__ unsigned long long g_phase[16];      // [kPhases] phases, [9] W x slowest, [10] workgroup steps
 __device__ unsigned long long g_phase_wave[16]; // pair-loop cycles per wave index
+// wall-clock start and exit of each workgroup of the last launch (the queue
+// tail: how long resident workgroups sit idle while the last items run)
+constexpr int kWgMax = 8192;
+__device__ unsigned long long g_wg_time[2][kWgMax];
 #define GD_PHASE(k)                                                                                                    \
     do {                                                                                                               \
         const unsigned long long t_ = clock64();                                                                       \
@@ -491,6 +495,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
 #ifdef GD_PHASE_PROFILE
     __shared__ unsigned long long s_ph[16], s_pw[16], s_pl[16];
     for (int i = threadIdx.x; i < 16; i += blockDim.x) s_ph[i] = s_pw[i] = 0;
+    if (threadIdx.x == 0 && blockIdx.x < kWgMax) g_wg_time[0][blockIdx.x] = wall_clock64();
     __syncthreads();
 #endif
     // waves of the item that have finished a step's pair loops; with PROG
@@ -1098,6 +1103,7 @@ __global__ void __launch_bounds__(1024, 6) mc_kernel(const DevParams *__restrict
         atomicAdd(&g_phase[threadIdx.x], s_ph[threadIdx.x]);
         atomicAdd(&g_phase_wave[threadIdx.x], s_pw[threadIdx.x]);
     }
+    if (threadIdx.x == 0 && blockIdx.x < kWgMax) g_wg_time[1][blockIdx.x] = wall_clock64();
 #endif
 }
 
@@ -1283,6 +1289,20 @@ extern "C" int gd_phase_profile(unsigned long long *out, int reset)
         if (hipMemcpyToSymbol(HIP_SYMBOL(gd::g_phase), z, sizeof z) != hipSuccess) return -3;
         if (hipMemcpyToSymbol(HIP_SYMBOL(gd::g_phase_wave), z, sizeof z) != hipSuccess) return -3;
     }
+    return 0;
+}
+
+// Wall-clock start (t[0..n)) and exit (t[n..2n)) of workgroups 0..n-1 of the
+// last launch, in wall-clock ticks (hipDeviceAttributeWallClockRate).
+// Exported by the phase-profile build only.
+extern "C" int gd_phase_profile_wg(unsigned long long *t, int n)
+{
+    if (!t || n <= 0 || n > gd::kWgMax) return -1;
+    const size_t bytes = sizeof(unsigned long long) * (size_t)n;
+    if (hipMemcpyFromSymbol(t, HIP_SYMBOL(gd::g_wg_time), bytes, 0) != hipSuccess) return -3;
+    if (hipMemcpyFromSymbol(t + n, HIP_SYMBOL(gd::g_wg_time), bytes, sizeof(unsigned long long) * gd::kWgMax) !=
+        hipSuccess)
+        return -3;
     return 0;
 }
 #endif

@@ -39,6 +39,8 @@ def main(argv=None):
     lib = L.load()
     lib.gd_phase_profile.argtypes = [C.POINTER(C.c_ulonglong), C.c_int]
     lib.gd_phase_profile.restype = C.c_int
+    lib.gd_phase_profile_wg.argtypes = [C.POINTER(C.c_ulonglong), C.c_int]
+    lib.gd_phase_profile_wg.restype = C.c_int
     shape = {}
     if args.shape:
         w, sw, sl = map(int, args.shape.split(","))
@@ -64,8 +66,14 @@ def main(argv=None):
         if lib.gd_phase_profile(buf, 1):
             raise RuntimeError("gd_phase_profile failed")
         W = dev.job_shape(jobs[0])[0]
+        plan = dev.batch_plan()
+        nwg = plan[1]
+        wg = (C.c_ulonglong * (2 * nwg))()
+        if lib.gd_phase_profile_wg(wg, nwg):
+            raise RuntimeError("gd_phase_profile_wg failed")
         print(f"{args.config}: {args.steps} steps in {ms:.2f} ms device, shape (waves, slot waves, slices, "
-              f"replicas) of the first job {dev.job_shape(jobs[0])}")
+              f"replicas) of the first job {dev.job_shape(jobs[0])}; last launch {plan[0]} tasks in {plan[2]} "
+              f"step slices over {nwg} workgroups")
     tot = sum(buf[k] for k in range(len(PHASES)))
     for k, name in enumerate(PHASES):
         print(f"  {name:32s} {100.0 * buf[k] / tot:6.2f} %")
@@ -74,6 +82,17 @@ def main(argv=None):
     print(f"  pair-loop imbalance (W x slowest / sum over waves, per workgroup step): {buf[9] / max(1, sum(waves)):.3f}"
           f" over {buf[10]} workgroup steps")
     print("  mean pair-loop cycles per wave index: " + " ".join(f"{w / n:.3g}" for w in waves))
+    # the last launch's workgroups on the wall clock: the span from the first
+    # start to the last exit, and the share of workgroup-time spent before a
+    # workgroup's start (ramp) and after its exit (queue tail)
+    t0 = [wg[i] for i in range(nwg)]
+    t1 = [wg[nwg + i] for i in range(nwg)]
+    a, b = min(t0), max(t1)
+    span = max(1, b - a)
+    ramp = sum(x - a for x in t0) / (nwg * span)
+    tail = sum(b - x for x in t1) / (nwg * span)
+    print(f"  last launch on the wall clock: {nwg} workgroups over {span} ticks; workgroup-time before the "
+          f"starts {100 * ramp:.2f} %, after the exits (queue tail) {100 * tail:.2f} %")
 
 
 if __name__ == "__main__":

@@ -111,11 +111,12 @@ def test_staging_refuses_malformed_loader_output(lib, kat_complex):
 
 def test_phase_profile_build_exports_its_reader():
     """The phase-profile build (a tracing build beside the product library)
-    exports gd_phase_profile; the product library does not carry the marks."""
+    exports its readers; the product library does not carry the marks."""
     from geauxdock_cs_v2_amd import build as B
     if not os.path.exists(B.PHASE_LIB):
         pytest.skip("phase-profile build not present (build.build_phase_profile)")
     out = subprocess.run(["nm", "-D", "--defined-only", B.PHASE_LIB], stdout=subprocess.PIPE, text=True).stdout
-    assert any(line.split()[-1] == "gd_phase_profile" for line in out.splitlines() if " T " in line)
+    names = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert {"gd_phase_profile", "gd_phase_profile_wg"} <= names
     prod = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], stdout=subprocess.PIPE, text=True).stdout
     assert "gd_phase_profile" not in prod
