@@ -462,6 +462,37 @@ def test_cached_batch_shapes_match_tuned(lib, ctx):
     assert cached == tuned
 
 
+def test_issue_priority_policies_multi_job_batch(lib, ctx):
+    """A batch of 12 screening ligands (15-50 atoms, per-job packings in one
+    launch) under the half-done boost and under the progress-balancing
+    kernel: 10 steps give the same replica states for every job, byte for
+    byte."""
+    from geauxdock_cs_v2_amd import synth
+    info = dict(synth.many_files("screen"))
+    info["ligands"] = info["ligands"][16:28]
+    cxs = synth.load_many(info, threads=8)
+    res = {}
+    try:
+        for policy in ("half", "progress"):
+            os.environ["GD_BOOST_POLICY"] = policy
+            say(f"case: 12 screening ligands batched 10 steps, policy {policy}")
+            t = C.c_int()
+            L.check(lib.gd_upload_target(ctx, C.byref(cxs[0].view), C.byref(t)))
+            jobs = []
+            for i, cx in enumerate(cxs):
+                j = C.c_int()
+                L.check(lib.gd_upload_job(ctx, t, C.byref(cx.view), 700 + i, C.byref(j)))
+                jobs.append(j.value)
+            _batch(lib, ctx, jobs, 10, comp=1)
+            res[policy] = [bytes(read_reps(lib, ctx, j, cx.size["n_rep"])) for j, cx in zip(jobs, cxs)]
+            for j in jobs:
+                L.check(lib.gd_free_job(ctx, j))
+            L.check(lib.gd_free_target(ctx, t))
+    finally:
+        os.environ.pop("GD_BOOST_POLICY", None)
+    assert res["progress"] == res["half"]
+
+
 def test_compensated_accumulation_within_tolerance(lib, ctx):
     """Kahan-compensated FP32 accumulation: 20 steps, final energies
     re-evaluated by the FP64 oracle at the GPU's final moves."""
